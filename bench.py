@@ -1,0 +1,148 @@
+"""Benchmark: DOF-stage updates/s of the RKDG hot path on one MI355X.
+
+Workload (default): isentropic vortex (gamma 1.4, beta 5, free stream rho=u=v=p=1) on the
+periodic [0,10]^2 square, 512 x 512 split quads with a seed-chosen diagonal per quad and
+interior vertices jittered by U(+-0.2 h) (524,288 triangles), P2 basis (6 modes per
+variable), 7-point volume / 3-point edge rules, Lax-Friedrichs flux, SSP-RK3 at CFL 0.3,
+FP64.  A step is one SSP-RK3 step (3 stage kernels) with the dt reduced on the device.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 10]
+
+Prints one JSON line.  value = sum E*Np*4*3*steps / wall time of the timed steps, with the
+state resident in HBM; the timed region is bracketed by device synchronisations and HIP
+events on the solver's stream (the stream the stage kernels are launched on).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_element_stage(Np):
+    # U read + U written + step-start state read in 2 of 3 stages, 14 geometry doubles,
+    # 3 neighbour ids; neighbour re-reads counted as cache hits
+    return 8 * 4 * Np * (1 + 1 + 2.0 / 3.0) + 112 + 12
+
+
+def workload(n=512, P=2):
+    from rkdg_heter_calculate_amd import Grid
+    g = Grid.splitquad(n, n, 0.0, 0.0, 10.0, 10.0, periodic=True, diagonal=1, jitter=0.2, seed=2024)
+    return g
+
+
+def cpu_baseline(g, P, steps, threads):
+    """The oracle (CPU restatement of the reference algorithm) on a bounded sample."""
+    from oracle import pyoracle as po
+    from rkdg_heter_calculate_amd import vortex_params
+    po.set_threads(threads)
+    ex, ey = g.element_coords()
+    o = po.Oracle(ex, ey, g.neighbour, g.shared_edge, g.boundary, P=P, flux=0, dt_policy=1, cfl=0.3)
+    o.project(1, vortex_params())
+    t0 = time.perf_counter()
+    o.advance(steps)
+    dt = time.perf_counter() - t0
+    Np = (P + 1) * (P + 2) // 2
+    o.close()
+    return g.n_elements * Np * 4 * 3 * steps / dt, dt
+
+
+def read_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512, help="split quads per side")
+    ap.add_argument("--degree", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the hot path on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is described, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from rkdg_heter_calculate_amd import Solver, IC_VORTEX, vortex_params
+
+    P = args.degree
+    Np = (P + 1) * (P + 2) // 2
+    g = workload(args.n, P)
+    E = g.n_elements
+    s = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=0.3)
+    s.project_ic(IC_VORTEX, vortex_params())
+    s.advance(args.warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    r = s.time_advance(args.steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    t_sim, nsteps = s.time()
+    err = s.error_vs_exact(IC_VORTEX, vortex_params(t=t_sim))
+
+    total_s = r["total_ms"] / 1e3
+    dofs = E * Np * 4
+    value = dofs * 3 * args.steps / total_s
+    B = algorithmic_bytes_per_element_stage(Np)
+    per_launch = B * E
+    achieved = per_launch / (r["stage_ms"] / 1e3) / 1e9
+    traffic = read_traffic(f"stage_p{P}_{args.n}x{args.n}_pmc.txt")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        v, secs = cpu_baseline(g, P, args.cpu_steps, threads)
+        cpu = {"value": v, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} SSP-RK3 steps of the same {E}-triangle P{P} mesh "
+                         f"({secs:.1f} s, OpenMP over elements)"}
+
+    out = {
+        "metric": "DOF-stage updates/s on 1 MI355X, FP64 vs FP32-MFMA at matched L2 error",
+        "value": value,
+        "unit": "DOF-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["total_ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered split-quad mesh, projected isentropic vortex)",
+        "config": {"workload": f"isentropic vortex, periodic [0,10]^2, {args.n}x{args.n} jittered split quads "
+                               f"({E} triangles), P{P}, LF, SSP-RK3 CFL 0.3, FP64",
+                   "elements": E, "degree": P, "dofs": dofs, "form": "plain FP64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"k_stage<{P},LF>", "bytes_per_launch": per_launch,
+                     "avg_launch_ms": r["stage_ms"]},
+        "cpu_baseline": cpu,
+        "wall_s": wall,
+        "l2_vs_exact": err["l2"],
+        "linf_vs_exact": err["linf"],
+        "sim_time": t_sim,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,141 @@
+"""ctypes binding of librkdg.so (declarations in include/rkdg.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or ``make -C
+rkdg_heter_calculate_amd/csrc``).  Loading fails loudly when it is missing: there is no
+fallback path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librkdg.so")
+
+c_int32 = ctypes.c_int32
+c_int64 = ctypes.c_int64
+c_double = ctypes.c_double
+P_double = ctypes.POINTER(ctypes.c_double)
+P_int32 = ctypes.POINTER(ctypes.c_int32)
+P_int64 = ctypes.POINTER(ctypes.c_int64)
+
+# status codes and enumerations (include/rkdg.h)
+OK, ERR_ARG, ERR_HIP, ERR_NONPHYSICAL, ERR_MESH, ERR_IO, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, -4, -5, -6, -7
+BC_FARFIELD, BC_OUTFLOW, BC_REFLECTION = 1, 2, 3
+FLUX_LF, FLUX_ROE = 0, 1
+LIMITER_NONE, LIMITER_TVB = 0, 1
+DT_REFERENCE, DT_MAX, DT_FIXED = 0, 1, 2
+PREC_FP64, PREC_FP32_MFMA, PREC_BF16_MFMA = 0, 1, 2
+BASIS_ELEMENT, BASIS_NATIVE = 0, 1
+IC_FREESTREAM, IC_VORTEX, IC_RIEMANN4 = 0, 1, 2
+
+
+class Mesh(ctypes.Structure):
+    _fields_ = [
+        ("n_vertices", c_int32), ("n_elements", c_int32),
+        ("vx", P_double), ("vy", P_double), ("tri", P_int32), ("neighbour", P_int32),
+        ("shared_edge", P_int32), ("boundary", P_int32), ("order", P_int32), ("dt_id", P_int32),
+    ]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("degree", c_int32), ("flux", c_int32), ("limiter", c_int32), ("dt_policy", c_int32),
+        ("reduction_threads", c_int32), ("precision", c_int32), ("device", c_int32), ("block", c_int32),
+        ("gamma", c_double), ("cfl", c_double), ("tvb_M", c_double), ("dt_fixed", c_double),
+        ("freestream", c_double * 4),
+    ]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [
+        ("n_vertices", c_int32), ("n_elements", c_int32),
+        ("vx", P_double), ("vy", P_double), ("tri", P_int32), ("neighbour", P_int32),
+        ("shared_edge", P_int32), ("boundary", P_int32), ("partition", P_int32),
+    ]
+
+
+# (name, restype, argtypes) for every entry point of include/rkdg.h
+SIGNATURES = [
+    ("rkdg_default_params", None, [ctypes.POINTER(Params)]),
+    ("rkdg_create", c_int32, [ctypes.POINTER(Mesh), ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
+    ("rkdg_destroy", c_int32, [ctypes.c_void_p]),
+    ("rkdg_set_state", c_int32, [ctypes.c_void_p, c_int32, P_double]),
+    ("rkdg_get_state", c_int32, [ctypes.c_void_p, c_int32, P_double]),
+    ("rkdg_project_ic", c_int32, [ctypes.c_void_p, c_int32, P_double, c_int32]),
+    ("rkdg_compute_dt", c_int32, [ctypes.c_void_p, P_double]),
+    ("rkdg_stage", c_int32, [ctypes.c_void_p, c_int32, c_double]),
+    ("rkdg_step", c_int32, [ctypes.c_void_p, P_double]),
+    ("rkdg_advance", c_int32, [ctypes.c_void_p, c_int32, c_double, P_int32, P_double]),
+    ("rkdg_residual", c_int32, [ctypes.c_void_p, P_double]),
+    ("rkdg_time", c_int32, [ctypes.c_void_p, P_double, P_int64]),
+    ("rkdg_get_flags", c_int32, [ctypes.c_void_p, P_int32]),
+    ("rkdg_sync", c_int32, [ctypes.c_void_p]),
+    ("rkdg_time_stage", c_int32, [ctypes.c_void_p, c_int32, c_int32, P_double]),
+    ("rkdg_time_advance", c_int32, [ctypes.c_void_p, c_int32, P_double]),
+    ("rkdg_error_vs_exact", c_int32, [ctypes.c_void_p, c_int32, P_double, c_int32, P_double]),
+    ("rkdg_info", c_int32, [ctypes.c_void_p, P_int64]),
+    ("rkdg_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
+    ("rkdg_strerror", ctypes.c_char_p, [c_int32]),
+    ("rkdg_batch_create", c_int32, [c_int32, ctypes.POINTER(Mesh), ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
+    ("rkdg_batch_destroy", c_int32, [ctypes.c_void_p]),
+    ("rkdg_batch_project_ic", c_int32, [ctypes.c_void_p, c_int32, c_int32, P_double, c_int32]),
+    ("rkdg_batch_set_state", c_int32, [ctypes.c_void_p, c_int32, c_int32, P_double]),
+    ("rkdg_batch_get_state", c_int32, [ctypes.c_void_p, c_int32, c_int32, P_double]),
+    ("rkdg_batch_advance", c_int32, [ctypes.c_void_p, c_int32, c_int32]),
+    ("rkdg_batch_time", c_int32, [ctypes.c_void_p, c_int32, P_double]),
+    ("rkdg_batch_sync", c_int32, [ctypes.c_void_p]),
+    ("rkdg_batch_autotune", c_int32, [ctypes.c_void_p, P_int32]),
+    ("rkdg_batch_info", c_int32, [ctypes.c_void_p, P_int64]),
+    ("rkdg_grid_splitquad", c_int32, [c_int32, c_int32, c_double, c_double, c_double, c_double, c_int32, c_int32,
+                                      c_double, ctypes.c_uint64, c_int32, ctypes.POINTER(Grid)]),
+    ("rkdg_grid_read", c_int32, [ctypes.c_char_p, ctypes.c_char_p, c_double, ctypes.POINTER(Grid)]),
+    ("rkdg_grid_locality_order", c_int32, [ctypes.POINTER(Grid), P_int32]),
+    ("rkdg_grid_free", None, [ctypes.POINTER(Grid)]),
+    ("rkdg_grid_last_error", ctypes.c_char_p, []),
+    ("rkdg_grid_as_mesh", None, [ctypes.POINTER(Grid), ctypes.POINTER(Mesh)]),
+    ("rkdg_write_plt", c_int32, [ctypes.c_char_p, ctypes.POINTER(Grid), c_int32, P_double, P_int32, c_double]),
+    ("rkdg_convert_basis", c_int32, [c_int32, c_int32, P_double, P_double, c_int32, P_double, P_double]),
+    ("rkdg_dump_state", c_int32, [ctypes.c_char_p, c_int32, c_int32, c_double, P_double]),
+    ("rkdg_load_state", c_int32, [ctypes.c_char_p, c_int32, c_int32, P_double, P_double]),
+    ("rkdg_config_get", c_int32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, c_int32]),
+]
+
+_lib = None
+
+
+class RkdgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"librkdg error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Returns the loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, ctx=None, what=""):
+    if rc != OK:
+        lib = load()
+        msg = lib.rkdg_last_error(ctx)
+        msg = msg.decode() if msg else ""
+        raise RkdgError(rc, f"{what}: {msg}")
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(P_double)
+
+
+def iptr(a):
+    return a.ctypes.data_as(P_int32)

@@ -1,0 +1,430 @@
+// Batching form: many independent meshes of uneven size advance together.
+//
+// All instances are concatenated into one SoA layout; each instance starts on a
+// 256-element boundary, so a workgroup tile (64, 128 or 256 elements) never spans two
+// instances and carries a single instance id (tile_mesh).  Every instance keeps its own
+// DevState, so its CFL dt is reduced and consumed on the device without any host
+// synchronisation.  One launch per RK stage advances every instance; the stage kernels
+// run persistently (grid capped at a few workgroups per CU, each looping over tiles),
+// and two full steps are captured once in a hipGraph and replayed.  The workgroup size
+// is chosen by timing (rkdg_batch_autotune).  Padding elements are valid dummy cells
+// (unit triangle, transmissive sides, uniform state at rest) excluded from the CFL
+// reduction.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "internal.h"
+
+using namespace rkdg;
+
+struct rkdg_batch {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    rkdg_params prm{};
+    int n = 0, P = 2, Np = 6, block = 256, persistent = 0, xcd_remap = 1;
+    long long ld = 0;           // total padded elements
+    std::vector<int> E, off;    // per instance
+    std::vector<std::vector<double>> ex, ey;
+    double* U[3] = {nullptr, nullptr, nullptr};
+    double* geo = nullptr;
+    int* nbr = nullptr;
+    unsigned char* dtflag = nullptr;
+    DevState* st = nullptr;
+    double* cfl = nullptr;
+    int* tile_mesh = nullptr;
+    int ntiles = 0;
+    long long steps = 0;
+    hipGraphExec_t graph2 = nullptr;
+    size_t bytes = 0;
+    std::string err;
+};
+
+static thread_local std::string g_berr;
+
+static int bfail(rkdg_batch* b, int code, const std::string& m) {
+    if (b) b->err = m;
+    g_berr = m;
+    return code;
+}
+
+#define BCHK(b, call)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (call);                                                                \
+        if (_e != hipSuccess) return bfail(b, RKDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+static int upload_tiles(rkdg_batch* b) {
+    b->ntiles = (int)(b->ld / b->block);
+    std::vector<int> tm(b->ntiles);
+    for (int i = 0; i < b->n; ++i) {
+        long long pad = (long long)(b->E[i] + 255) / 256 * 256;
+        for (long long t = b->off[i] / b->block; t < (b->off[i] + pad) / b->block; ++t)
+            tm[t] = (i << 1) | (t * b->block == b->off[i] ? 1 : 0);
+    }
+    if (b->tile_mesh) hipFree(b->tile_mesh);
+    BCHK(b, hipMalloc(&b->tile_mesh, sizeof(int) * tm.size()));
+    BCHK(b, hipMemcpy(b->tile_mesh, tm.data(), sizeof(int) * tm.size(), hipMemcpyHostToDevice));
+    if (b->graph2) { hipGraphExecDestroy(b->graph2); b->graph2 = nullptr; }
+    return RKDG_OK;
+}
+
+static StageArgs bargs(rkdg_batch* b, int stage, int slot) {
+    StageArgs a{};
+    const int u_in = stage, u_out = stage == 2 ? 0 : stage + 1;
+    a.U = b->U[u_in];
+    a.U0 = b->U[0];
+    a.Uout = b->U[u_out];
+    a.geo = b->geo;
+    a.nbr = b->nbr;
+    a.dtflag = b->dtflag;
+    a.st = b->st;
+    a.tile_mesh = b->tile_mesh;
+    a.st_batch = b->st;
+    a.cfl_batch = b->cfl;
+    a.ld = b->ld;
+    a.E = (int)b->ld;
+    a.stage = stage;
+    a.slot = slot;
+    a.dt_mode = b->prm.dt_policy == RKDG_DT_FIXED ? DTM_FIXED : DTM_DEVICE;
+    a.dt_host = b->prm.dt_fixed;
+    a.xcd_remap = b->xcd_remap;
+    a.ntiles = b->ntiles;
+    a.persistent = b->persistent;
+    a.cfl = b->prm.cfl;
+    a.gamma = b->prm.gamma;
+    a.t_end = 0.0;
+    for (int v = 0; v < 4; ++v) a.fs[v] = b->prm.freestream[v];
+    return a;
+}
+
+static int bstep(rkdg_batch* b, int slot) {
+    for (int s = 0; s < 3; ++s) {
+        int rc = launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, bargs(b, s, slot), b->stream);
+        if (rc) return bfail(b, RKDG_ERR_HIP, std::string("batch stage: ") + hip_error_string(rc));
+    }
+    return RKDG_OK;
+}
+
+static int bwavespeed_all(rkdg_batch* b) {
+    int rc = launch_state_init(b->st, 0, 0.0, b->n, b->stream);
+    if (rc) return bfail(b, RKDG_ERR_HIP, hip_error_string(rc));
+    StageArgs a = bargs(b, 0, 0);
+    a.U = b->U[0];
+    rc = launch_wavespeed(b->P, a, b->block, b->stream);
+    if (rc) return bfail(b, RKDG_ERR_HIP, hip_error_string(rc));
+    b->steps = 0;
+    return RKDG_OK;
+}
+
+extern "C" {
+
+int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* params, rkdg_batch** out) {
+    if (n < 1 || !meshes || !params || !out) return bfail(nullptr, RKDG_ERR_ARG, "batch arguments");
+    const rkdg_params& p0 = params[0];
+    if (p0.degree < 1 || p0.degree > 3) return bfail(nullptr, RKDG_ERR_ARG, "degree");
+    if (p0.limiter != RKDG_LIMITER_NONE) return bfail(nullptr, RKDG_ERR_ARG, "the batching form runs without limiter");
+    if (p0.precision != RKDG_PREC_FP64) return bfail(nullptr, RKDG_ERR_ARG, "precision form not available in this build");
+    for (int i = 1; i < n; ++i)
+        if (params[i].degree != p0.degree || params[i].flux != p0.flux || params[i].gamma != p0.gamma ||
+            params[i].dt_policy != p0.dt_policy || params[i].precision != p0.precision)
+            return bfail(nullptr, RKDG_ERR_ARG, "instances must share degree, flux, gamma, dt policy, precision");
+    auto b = new rkdg_batch();
+    b->prm = p0;
+    b->n = n;
+    b->P = p0.degree;
+    b->Np = np_of(b->P);
+    b->block = p0.block > 0 ? p0.block : 256;
+    b->xcd_remap = 1;
+    b->E.resize(n);
+    b->off.resize(n);
+    b->ex.resize(n);
+    b->ey.resize(n);
+    long long tot = 0;
+    for (int i = 0; i < n; ++i) {
+        b->E[i] = meshes[i].n_elements;
+        if (b->E[i] < 1 || !meshes[i].tri || !meshes[i].neighbour) { delete b; return bfail(nullptr, RKDG_ERR_ARG, "mesh " + std::to_string(i)); }
+        b->off[i] = (int)tot;
+        tot += (long long)(b->E[i] + 255) / 256 * 256;
+    }
+    if (tot > (1LL << 29)) { delete b; return bfail(nullptr, RKDG_ERR_ARG, "too many elements"); }
+    b->ld = tot;
+    const long long ld = tot;
+    std::vector<double> hgeo((size_t)GEO_ROWS * ld);
+    std::vector<int> hnbr((size_t)3 * ld, -RKDG_BC_OUTFLOW);
+    std::vector<unsigned char> hflag(ld, 0);
+    std::vector<double> hU((size_t)4 * b->Np * ld, 0.0);
+    // dummy padding cell: unit right triangle, fluid at rest
+    {
+        double x[3] = {0, 1, 0}, y[3] = {0, 0, 1}, g[GEO_ROWS];
+        element_geometry(x, y, g);
+        for (long long p = 0; p < ld; ++p) {
+            for (int r = 0; r < GEO_ROWS; ++r) hgeo[(size_t)r * ld + p] = g[r];
+            hU[(size_t)(0 * b->Np) * ld + p] = 1.0;
+            hU[(size_t)(3 * b->Np) * ld + p] = 1.0 / (p0.gamma - 1.0);
+        }
+    }
+    std::string merr;
+    for (int i = 0; i < n && merr.empty(); ++i) {
+        const rkdg_mesh& m = meshes[i];
+        const int E = b->E[i], off = b->off[i];
+        b->ex[i].resize((size_t)3 * E);
+        b->ey[i].resize((size_t)3 * E);
+        std::vector<int32_t> se((size_t)3 * E);
+        if (m.shared_edge) std::memcpy(se.data(), m.shared_edge, sizeof(int32_t) * 3 * E);
+        else if (build_shared_edges(E, m.tri, m.neighbour, se.data(), merr)) break;
+        for (int e = 0; e < E; ++e) {
+            for (int k = 0; k < 3; ++k) {
+                int v = m.tri[3 * e + k];
+                b->ex[i][3 * e + k] = m.vx[v];
+                b->ey[i][3 * e + k] = m.vy[v];
+            }
+            double g[GEO_ROWS];
+            element_geometry(&b->ex[i][3 * e], &b->ey[i][3 * e], g);
+            if (!(g[4] > 0)) { merr = "degenerate element"; break; }
+            for (int r = 0; r < GEO_ROWS; ++r) hgeo[(size_t)r * ld + off + e] = g[r];
+            for (int k = 0; k < 3; ++k) {
+                int nb = m.neighbour[3 * e + k];
+                if (nb >= 0) hnbr[(size_t)k * ld + off + e] = ((off + nb) << 2) | se[3 * e + k];
+                else hnbr[(size_t)k * ld + off + e] = -(m.boundary ? m.boundary[3 * e + k] : RKDG_BC_OUTFLOW);
+            }
+            int id = m.dt_id ? m.dt_id[e] : e;
+            bool inc = true;
+            if (p0.dt_policy == RKDG_DT_REFERENCE) {
+                int R = p0.reduction_threads > 0 ? p0.reduction_threads : 512;
+                if (R >= 128) inc = (id % 128) < 64;
+            }
+            hflag[off + e] = inc;
+        }
+    }
+    if (!merr.empty()) { delete b; return bfail(nullptr, RKDG_ERR_MESH, merr); }
+    std::vector<double> hcfl(n);
+    for (int i = 0; i < n; ++i) hcfl[i] = params[i].cfl;
+
+    b->dev = p0.device;
+    BCHK(b, hipSetDevice(b->dev));
+    BCHK(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    size_t ub = sizeof(double) * hU.size();
+    for (int k = 0; k < 3; ++k) {
+        BCHK(b, hipMalloc(&b->U[k], ub));
+        BCHK(b, hipMemcpy(b->U[k], hU.data(), ub, hipMemcpyHostToDevice));
+    }
+    BCHK(b, hipMalloc(&b->geo, sizeof(double) * hgeo.size()));
+    BCHK(b, hipMalloc(&b->nbr, sizeof(int) * hnbr.size()));
+    BCHK(b, hipMalloc(&b->dtflag, hflag.size()));
+    BCHK(b, hipMalloc(&b->st, sizeof(DevState) * n));
+    BCHK(b, hipMalloc(&b->cfl, sizeof(double) * n));
+    BCHK(b, hipMemcpy(b->geo, hgeo.data(), sizeof(double) * hgeo.size(), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpy(b->nbr, hnbr.data(), sizeof(int) * hnbr.size(), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpy(b->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpy(b->cfl, hcfl.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    b->bytes = 3 * ub + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size();
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
+    b->persistent = cus * 8;
+    int rc = upload_tables(b->P, ref_basis(b->P).tab);
+    if (rc) { rkdg_batch_destroy(b); return bfail(nullptr, RKDG_ERR_HIP, "table upload"); }
+    rc = upload_tiles(b);
+    if (rc) { std::string m2 = b->err; rkdg_batch_destroy(b); return bfail(nullptr, rc, m2); }
+    rc = bwavespeed_all(b);
+    if (rc) { std::string m2 = b->err; rkdg_batch_destroy(b); return bfail(nullptr, rc, m2); }
+    BCHK(b, hipStreamSynchronize(b->stream));
+    *out = b;
+    return RKDG_OK;
+}
+
+int rkdg_batch_destroy(rkdg_batch* b) {
+    if (!b) return RKDG_OK;
+    hipSetDevice(b->dev);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->graph2) hipGraphExecDestroy(b->graph2);
+    for (int k = 0; k < 3; ++k) hipFree(b->U[k]);
+    hipFree(b->geo); hipFree(b->nbr); hipFree(b->dtflag); hipFree(b->st); hipFree(b->cfl); hipFree(b->tile_mesh);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return RKDG_OK;
+}
+
+int rkdg_batch_set_state(rkdg_batch* b, int32_t i, int32_t basis, const double* U) {
+    if (!b || i < 0 || i >= b->n || !U) return bfail(b, RKDG_ERR_ARG, "batch_set_state");
+    BCHK(b, hipSetDevice(b->dev));
+    const int E = b->E[i], Np = b->Np, P = b->P;
+    std::vector<double> h((size_t)4 * Np * E);
+    parallel_for(E, [&](int a0, int a1) {
+        double in[4 * MAXNP], o[4 * MAXNP];
+        for (int e = a0; e < a1; ++e) {
+            for (int v = 0; v < 4; ++v)
+                for (int k = 0; k < Np; ++k) in[v * Np + k] = U[((size_t)v * Np + k) * E + e];
+            if (basis == RKDG_BASIS_ELEMENT) element_to_native(P, &b->ex[i][3 * e], &b->ey[i][3 * e], in, o);
+            else std::memcpy(o, in, sizeof(double) * 4 * Np);
+            for (int r = 0; r < 4 * Np; ++r) h[(size_t)r * E + e] = o[r];
+        }
+    });
+    // rows of E doubles into rows of ld doubles, starting at the instance offset
+    BCHK(b, hipMemcpy2DAsync(b->U[0] + b->off[i], sizeof(double) * b->ld, h.data(), sizeof(double) * E,
+                             sizeof(double) * E, 4 * Np, hipMemcpyHostToDevice, b->stream));
+    int rc = bwavespeed_all(b);
+    if (rc) return rc;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    return RKDG_OK;
+}
+
+int rkdg_batch_get_state(rkdg_batch* b, int32_t i, int32_t basis, double* U) {
+    if (!b || i < 0 || i >= b->n || !U) return bfail(b, RKDG_ERR_ARG, "batch_get_state");
+    BCHK(b, hipSetDevice(b->dev));
+    const int E = b->E[i], Np = b->Np, P = b->P;
+    std::vector<double> h((size_t)4 * Np * E);
+    BCHK(b, hipMemcpy2DAsync(h.data(), sizeof(double) * E, b->U[0] + b->off[i], sizeof(double) * b->ld,
+                             sizeof(double) * E, 4 * Np, hipMemcpyDeviceToHost, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    parallel_for(E, [&](int a0, int a1) {
+        double in[4 * MAXNP], o[4 * MAXNP];
+        for (int e = a0; e < a1; ++e) {
+            for (int r = 0; r < 4 * Np; ++r) in[r] = h[(size_t)r * E + e];
+            if (basis == RKDG_BASIS_ELEMENT) native_to_element(P, &b->ex[i][3 * e], &b->ey[i][3 * e], in, o);
+            else std::memcpy(o, in, sizeof(double) * 4 * Np);
+            for (int v = 0; v < 4; ++v)
+                for (int k = 0; k < Np; ++k) U[((size_t)v * Np + k) * E + e] = o[v * Np + k];
+        }
+    });
+    return RKDG_OK;
+}
+
+int rkdg_batch_project_ic(rkdg_batch* b, int32_t i, int32_t kind, const double* p, int32_t np) {
+    if (!b || i < 0 || i >= b->n || !p) return bfail(b, RKDG_ERR_ARG, "batch_project_ic");
+    int need = kind == RKDG_IC_FREESTREAM ? 5 : kind == RKDG_IC_VORTEX ? 6 : kind == RKDG_IC_RIEMANN4 ? 19 : -1;
+    if (need < 0 || np < need) return bfail(b, RKDG_ERR_ARG, "initial condition");
+    const int E = b->E[i], Np = b->Np, P = b->P;
+    std::vector<double> U((size_t)4 * Np * E, 0.0);
+    parallel_for(E, [&](int a0, int a1) {
+        double d[4 * MAXNP];
+        for (int e = a0; e < a1; ++e) {
+            if (kind == RKDG_IC_FREESTREAM) {
+                double s[4];
+                ic_state(kind, p, np, 0, 0, s);
+                for (int r = 0; r < 4 * Np; ++r) d[r] = 0;
+                for (int v = 0; v < 4; ++v) d[v * Np] = s[v];
+            } else {
+                project_native(P, &b->ex[i][3 * e], &b->ey[i][3 * e], kind, p, np, d);
+            }
+            for (int v = 0; v < 4; ++v)
+                for (int k = 0; k < Np; ++k) U[((size_t)v * Np + k) * E + e] = d[v * Np + k];
+        }
+    });
+    return rkdg_batch_set_state(b, i, RKDG_BASIS_NATIVE, U.data());
+}
+
+int rkdg_batch_advance(rkdg_batch* b, int32_t nsteps, int32_t use_graph) {
+    if (!b || nsteps < 0) return bfail(b, RKDG_ERR_ARG, "batch_advance");
+    BCHK(b, hipSetDevice(b->dev));
+    int done = 0;
+    if (use_graph && (b->steps & 1) && nsteps > 0) {
+        int rc = bstep(b, 1);
+        if (rc) return rc;
+        b->steps++;
+        done++;
+    }
+    if (use_graph && nsteps - done >= 2) {
+        if (!b->graph2) {
+            hipGraph_t g;
+            BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            int rc = bstep(b, 0);
+            if (!rc) rc = bstep(b, 1);
+            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (rc) return rc;
+            if (e2 != hipSuccess) return bfail(b, RKDG_ERR_HIP, std::string("capture: ") + hipGetErrorString(e2));
+            hipError_t e3 = hipGraphInstantiate(&b->graph2, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e3 != hipSuccess) { b->graph2 = nullptr; return bfail(b, RKDG_ERR_HIP, hipGetErrorString(e3)); }
+        }
+        while (nsteps - done >= 2) {
+            BCHK(b, hipGraphLaunch(b->graph2, b->stream));
+            b->steps += 2;
+            done += 2;
+        }
+    }
+    while (done < nsteps) {
+        int rc = bstep(b, (int)(b->steps & 1));
+        if (rc) return rc;
+        b->steps++;
+        done++;
+    }
+    return RKDG_OK;
+}
+
+int rkdg_batch_sync(rkdg_batch* b) {
+    if (!b) return RKDG_ERR_ARG;
+    BCHK(b, hipSetDevice(b->dev));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    std::vector<DevState> h(b->n);
+    BCHK(b, hipMemcpy(h.data(), b->st, sizeof(DevState) * b->n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->n; ++i)
+        if (h[i].err_pos != INT_MAX)
+            return bfail(b, RKDG_ERR_NONPHYSICAL, "non-physical state in instance " + std::to_string(i));
+    return RKDG_OK;
+}
+
+int rkdg_batch_time(rkdg_batch* b, int32_t i, double* t) {
+    if (!b || i < 0 || i >= b->n || !t) return bfail(b, RKDG_ERR_ARG, "batch_time");
+    BCHK(b, hipSetDevice(b->dev));
+    DevState h;
+    BCHK(b, hipMemcpyAsync(&h, b->st + i, sizeof h, hipMemcpyDeviceToHost, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    *t = h.t[b->steps & 1];
+    return RKDG_OK;
+}
+
+int rkdg_batch_autotune(rkdg_batch* b, int32_t* best_block) {
+    if (!b) return RKDG_ERR_ARG;
+    BCHK(b, hipSetDevice(b->dev));
+    const int cands[3] = {64, 128, 256};
+    const int pers[3] = {0, 4, 8};  // 0: one workgroup per tile; else workgroups per CU
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
+    float best = 1e30f;
+    int bb = b->block, bp = b->persistent;
+    hipEvent_t e0, e1;
+    BCHK(b, hipEventCreate(&e0));
+    BCHK(b, hipEventCreate(&e1));
+    for (int c : cands)
+        for (int pw : pers) {
+            b->block = c;
+            b->persistent = pw ? cus * pw * 256 / c : 0;
+            int rc = upload_tiles(b);
+            if (rc) return rc;
+            // stage 1 reads buffers 1 and 0 and writes buffer 2: repeatable without
+            // changing the state
+            StageArgs a = bargs(b, 1, (int)(b->steps & 1));
+            launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, a, b->stream);
+            BCHK(b, hipEventRecord(e0, b->stream));
+            for (int r = 0; r < 10; ++r) launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, a, b->stream);
+            BCHK(b, hipEventRecord(e1, b->stream));
+            BCHK(b, hipEventSynchronize(e1));
+            float ms;
+            BCHK(b, hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; bb = c; bp = b->persistent; }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    b->block = bb;
+    b->persistent = bp;
+    int rc = upload_tiles(b);
+    if (rc) return rc;
+    if (best_block) *best_block = bb;
+    return RKDG_OK;
+}
+
+int rkdg_batch_info(rkdg_batch* b, int64_t out[4]) {
+    if (!b || !out) return RKDG_ERR_ARG;
+    long long tot = 0;
+    for (int e : b->E) tot += e;
+    out[0] = b->n; out[1] = tot; out[2] = b->block; out[3] = (int64_t)b->bytes;
+    return RKDG_OK;
+}
+
+}  // extern "C"

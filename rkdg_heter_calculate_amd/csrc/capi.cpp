@@ -1,0 +1,764 @@
+// C ABI of librkdg: context set-up, device layout, state exchange and the time loop.
+//
+// Device layout (one context, E elements padded to ld = multiple of 256):
+//   U[3]    three FP64 state buffers [4 vars][Np modes][ld]: the step-start state and
+//           the two intermediate SSP-RK3 stages (neighbours read the previous stage while
+//           the next is written, so stages cannot update in place)
+//   geo     [14][ld] inverse Jacobian, area, edge lengths, outward normals
+//   nbr     [3][ld] int32 packed (device position << 2 | neighbour's edge), or -tag
+//   dtflag  [ld] uint8, 1 if the element takes part in the CFL reduction
+//   lim     [7][ld], pairs [ld] (TVB limiter geometry, only with the limiter)
+//   st      DevState (wave-speed maxima, time, dt per step parity)
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <atomic>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+#include "internal.h"
+
+namespace rkdg {
+
+void parallel_for(int n, const std::function<void(int, int)>& body) {
+    unsigned hw = std::thread::hardware_concurrency();
+    int nt = (int)std::min<unsigned>(hw ? hw : 1, 16);
+    if (const char* s = std::getenv("RKDG_HOST_THREADS")) nt = std::max(1, std::atoi(s));
+    if (n < 4096 || nt <= 1) { body(0, n); return; }
+    std::vector<std::thread> th;
+    int chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        int a = t * chunk, b = std::min(n, a + chunk);
+        if (a >= b) break;
+        th.emplace_back(body, a, b);
+    }
+    for (auto& x : th) x.join();
+}
+
+}  // namespace rkdg
+
+using namespace rkdg;
+
+struct rkdg_ctx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    rkdg_params prm{};
+    int P = 2, Np = 6, Nqv = 7, Nqe = 3, E = 0, block = 256, ntiles = 0, xcd_remap = 1;
+    long long ld = 0;
+    std::vector<int32_t> order, pos;  // order[p] = caller id at device position p
+    std::vector<double> ex, ey;       // [E][3] element vertex coordinates (caller order)
+    double* U[3] = {nullptr, nullptr, nullptr};
+    double* geo = nullptr;
+    int* nbr = nullptr;
+    unsigned char* dtflag = nullptr;
+    double* lim = nullptr;
+    int* pairs = nullptr;
+    int* flags = nullptr;
+    DevState* st = nullptr;
+    int cur = 0;            // buffer holding the latest state
+    int next_stage = 0;     // rkdg_stage protocol
+    long long steps = 0;    // steps launched (host count; parity of the step slot)
+    bool have_state = false;
+    size_t bytes = 0;
+    hipGraphExec_t graph2 = nullptr;  // two steps (slot 0 then 1)
+    std::string err;
+};
+
+static thread_local std::string g_err;
+
+static int fail(rkdg_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                     \
+    do {                                                                                      \
+        hipError_t _e = (call);                                                               \
+        if (_e != hipSuccess) return fail(ctx, RKDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+static int env_int(const char* name, int dflt) {
+    const char* s = std::getenv(name);
+    return s ? std::atoi(s) : dflt;
+}
+
+static StageArgs make_args(rkdg_ctx* c, int stage, int slot, int dt_mode, double dt, double t_end) {
+    StageArgs a{};
+    const int u_in = stage == 0 ? 0 : stage;     // stage s reads buffer s (0, 1, 2)
+    const int u_out = stage == 2 ? 0 : stage + 1;
+    a.U = c->U[u_in];
+    a.U0 = c->U[0];
+    a.Uout = c->U[u_out];
+    a.geo = c->geo;
+    a.nbr = c->nbr;
+    a.dtflag = c->dtflag;
+    a.st = c->st;
+    a.ld = c->ld;
+    a.E = c->E;
+    a.stage = stage;
+    a.slot = slot;
+    a.dt_mode = dt_mode;
+    a.dt_host = dt;
+    a.xcd_remap = c->xcd_remap;
+    a.ntiles = c->ntiles;
+    a.cfl = c->prm.cfl;
+    a.gamma = c->prm.gamma;
+    a.t_end = t_end;
+    for (int v = 0; v < 4; ++v) a.fs[v] = c->prm.freestream[v];
+    return a;
+}
+
+static int launch_one_stage(rkdg_ctx* c, int s, int slot, int dt_mode, double dt, double t_end) {
+    StageArgs a = make_args(c, s, slot, dt_mode, dt, t_end);
+    int rc = launch_stage(c->P, c->prm.flux, c->prm.precision, c->block, a, c->stream);
+    if (rc) return fail(c, RKDG_ERR_HIP, std::string("stage kernel: ") + hip_error_string(rc));
+    if (c->prm.limiter == RKDG_LIMITER_TVB) {
+        LimArgs la{};
+        la.U = a.Uout; la.lim = c->lim; la.pairs = c->pairs; la.nbr = c->nbr; la.flags = c->flags;
+        la.ld = c->ld; la.E = c->E; la.M = c->prm.tvb_M;
+        rc = launch_limiter(c->P, la, c->block, c->stream);
+        if (rc) return fail(c, RKDG_ERR_HIP, std::string("limiter kernel: ") + hip_error_string(rc));
+    }
+    return RKDG_OK;
+}
+
+static int launch_step(rkdg_ctx* c, int slot, double t_end) {
+    int dt_mode = c->prm.dt_policy == RKDG_DT_FIXED ? DTM_FIXED : DTM_DEVICE;
+    for (int s = 0; s < 3; ++s) {
+        int rc = launch_one_stage(c, s, slot, dt_mode, c->prm.dt_fixed, t_end);
+        if (rc) return rc;
+    }
+    return RKDG_OK;
+}
+
+static int refresh_wavespeed(rkdg_ctx* c, int slot, double t) {
+    int rc = launch_state_init(c->st, slot, t, 1, c->stream);
+    if (rc) return fail(c, RKDG_ERR_HIP, hip_error_string(rc));
+    StageArgs a = make_args(c, 0, slot, DTM_DEVICE, 0.0, 0.0);
+    a.U = c->U[c->cur];
+    rc = launch_wavespeed(c->P, a, c->block, c->stream);
+    if (rc) return fail(c, RKDG_ERR_HIP, hip_error_string(rc));
+    return RKDG_OK;
+}
+
+static int check_physical(rkdg_ctx* c) {
+    DevState h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (h.err_pos != INT_MAX) {
+        int id = (h.err_pos >= 0 && h.err_pos < c->E) ? c->order[h.err_pos] : h.err_pos;
+        return fail(c, RKDG_ERR_NONPHYSICAL, "non-positive density or pressure in element " + std::to_string(id));
+    }
+    return RKDG_OK;
+}
+
+extern "C" {
+
+void rkdg_default_params(rkdg_params* p) {
+    std::memset(p, 0, sizeof *p);
+    p->degree = 2;
+    p->flux = RKDG_FLUX_LF;
+    p->limiter = RKDG_LIMITER_NONE;
+    p->dt_policy = RKDG_DT_REFERENCE;
+    p->reduction_threads = 512;
+    p->precision = RKDG_PREC_FP64;
+    p->device = 0;
+    p->block = 0;
+    p->gamma = 1.4;
+    p->cfl = 0.18;
+    p->tvb_M = 50.0;
+    p->dt_fixed = 0.0;
+    // free stream of src/rkdgadvance.cpp:31-34 with the defaults mach 0.4, alpha 5 deg
+    double g = 1.4, ut = std::sqrt(g * 1.0 / 1.0) * 0.4, al = 5.0 * std::atan(1.0) * 4 / 180;
+    p->freestream[0] = 1.0;
+    p->freestream[1] = ut * std::cos(al);
+    p->freestream[2] = ut * std::sin(al);
+    p->freestream[3] = 0.5 * ut * ut + 1.0 / (g - 1);
+}
+
+const char* rkdg_strerror(int32_t code) {
+    switch (code) {
+        case RKDG_OK: return "ok";
+        case RKDG_ERR_ARG: return "invalid argument";
+        case RKDG_ERR_HIP: return "HIP runtime error";
+        case RKDG_ERR_NONPHYSICAL: return "non-physical state";
+        case RKDG_ERR_MESH: return "inconsistent mesh";
+        case RKDG_ERR_IO: return "I/O error";
+        case RKDG_ERR_STATE: return "call out of order";
+        case RKDG_ERR_NOMEM: return "out of memory";
+        default: return "unknown error";
+    }
+}
+
+const char* rkdg_last_error(rkdg_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
+    if (!m || !prm || !out || m->n_elements < 1 || !m->vx || !m->vy || !m->tri || !m->neighbour)
+        return fail(nullptr, RKDG_ERR_ARG, "mesh arrays missing");
+    if (prm->degree < 1 || prm->degree > 3) return fail(nullptr, RKDG_ERR_ARG, "degree must be 1, 2 or 3");
+    if (prm->flux != RKDG_FLUX_LF && prm->flux != RKDG_FLUX_ROE) return fail(nullptr, RKDG_ERR_ARG, "flux");
+    if (prm->precision != RKDG_PREC_FP64) return fail(nullptr, RKDG_ERR_ARG, "precision form not available in this build");
+    if (!(prm->cfl > 0) && prm->dt_policy != RKDG_DT_FIXED) return fail(nullptr, RKDG_ERR_ARG, "cfl");
+    if (prm->dt_policy == RKDG_DT_FIXED && !(prm->dt_fixed > 0)) return fail(nullptr, RKDG_ERR_ARG, "dt_fixed");
+    if (m->n_elements > (1 << 29) - 1) return fail(nullptr, RKDG_ERR_ARG, "too many elements");
+    auto c = new rkdg_ctx();
+    c->prm = *prm;
+    c->P = prm->degree; c->Np = np_of(c->P); c->Nqv = nqv_of(c->P); c->Nqe = nqe_of(c->P);
+    c->E = m->n_elements;
+    c->block = prm->block > 0 ? prm->block : env_int("RKDG_BLOCK", 256);
+    if (c->block % 64 || c->block < 64 || c->block > 256) { delete c; return fail(nullptr, RKDG_ERR_ARG, "block must be 64..256, multiple of 64"); }
+    c->xcd_remap = env_int("RKDG_XCD_REMAP", 1);
+    c->ntiles = (c->E + c->block - 1) / c->block;
+    c->ld = (long long)((c->E + 255) / 256) * 256;
+    const int E = c->E;
+    const long long ld = c->ld;
+
+    // locality permutation
+    c->order.resize(E);
+    c->pos.resize(E);
+    for (int p = 0; p < E; ++p) c->order[p] = m->order ? m->order[p] : p;
+    std::vector<char> seen(E, 0);
+    for (int p = 0; p < E; ++p) {
+        int id = c->order[p];
+        if (id < 0 || id >= E || seen[id]) { delete c; return fail(nullptr, RKDG_ERR_ARG, "order is not a permutation"); }
+        seen[id] = 1;
+        c->pos[id] = p;
+    }
+    // element coordinates (caller order)
+    c->ex.resize((size_t)3 * E);
+    c->ey.resize((size_t)3 * E);
+    for (int e = 0; e < E; ++e)
+        for (int k = 0; k < 3; ++k) {
+            int v = m->tri[3 * e + k];
+            if (v < 0 || v >= m->n_vertices) { delete c; return fail(nullptr, RKDG_ERR_MESH, "vertex id out of range"); }
+            c->ex[3 * e + k] = m->vx[v];
+            c->ey[3 * e + k] = m->vy[v];
+        }
+    // shared edges
+    std::vector<int32_t> se((size_t)3 * E);
+    if (m->shared_edge) {
+        std::memcpy(se.data(), m->shared_edge, sizeof(int32_t) * 3 * E);
+    } else {
+        std::string err;
+        int rc = build_shared_edges(E, m->tri, m->neighbour, se.data(), err);
+        if (rc) { delete c; return fail(nullptr, rc, err); }
+    }
+    // host images of the device arrays (device order)
+    std::vector<double> hgeo((size_t)GEO_ROWS * ld, 0.0);
+    std::vector<int> hnbr((size_t)3 * ld, -RKDG_BC_OUTFLOW);
+    std::vector<unsigned char> hflag(ld, 0);
+    std::string merr;
+    for (int p = 0; p < E; ++p) {
+        int e = c->order[p];
+        double g[GEO_ROWS];
+        element_geometry(&c->ex[3 * e], &c->ey[3 * e], g);
+        if (!(g[4] > 0)) { merr = "degenerate element " + std::to_string(e); break; }
+        for (int r = 0; r < GEO_ROWS; ++r) hgeo[(size_t)r * ld + p] = g[r];
+        for (int k = 0; k < 3; ++k) {
+            int n = m->neighbour[3 * e + k];
+            if (n >= 0) {
+                if (n >= E || m->neighbour[3 * n + se[3 * e + k]] != e) { merr = "neighbour relation not symmetric at element " + std::to_string(e); break; }
+                hnbr[(size_t)k * ld + p] = (c->pos[n] << 2) | se[3 * e + k];
+            } else {
+                int t = m->boundary ? m->boundary[3 * e + k] : RKDG_BC_OUTFLOW;
+                if (t != RKDG_BC_FARFIELD && t != RKDG_BC_OUTFLOW && t != RKDG_BC_REFLECTION) { merr = "unknown boundary tag"; break; }
+                hnbr[(size_t)k * ld + p] = -t;
+            }
+        }
+        int id = m->dt_id ? m->dt_id[e] : e;
+        bool inc = true;
+        if (prm->dt_policy == RKDG_DT_REFERENCE) {
+            // kernel_getTimeStep's tree stops at index > 64 and its unrolled tail reads
+            // mx[0..63] only: threads tid % 128 >= 64 drop out when blockDim >= 128
+            int R = prm->reduction_threads > 0 ? prm->reduction_threads : 512;
+            if (R >= 128) inc = (id % 128) < 64;
+        }
+        hflag[p] = inc ? 1 : 0;
+        if (!merr.empty()) break;
+    }
+    if (!merr.empty()) { delete c; return fail(nullptr, RKDG_ERR_MESH, merr); }
+    // limiter geometry
+    std::vector<double> hlim;
+    std::vector<int> hpairs;
+    if (prm->limiter == RKDG_LIMITER_TVB) {
+        hlim.assign((size_t)LIM_ROWS * ld, 0.0);
+        hpairs.assign(ld, 0);
+        for (int p = 0; p < E; ++p) {
+            int e = c->order[p];
+            const double* x = &c->ex[3 * e];
+            const double* y = &c->ey[3 * e];
+            double bx = (x[0] + x[1] + x[2]) / 3, by = (y[0] + y[1] + y[2]) / 3;
+            double nbx[3], nby[3];
+            for (int k = 0; k < 3; ++k) {
+                int a = (k + 1) % 3, b = (k + 2) % 3;
+                double mx = 0.5 * (x[a] + x[b]), my = 0.5 * (y[a] + y[b]);
+                int n = m->neighbour[3 * e + k];
+                if (n >= 0) {
+                    const double* xn = &c->ex[3 * n];
+                    const double* yn = &c->ey[3 * n];
+                    int sn = se[3 * e + k];
+                    double nmx = 0.5 * (xn[(sn + 1) % 3] + xn[(sn + 2) % 3]);
+                    double nmy = 0.5 * (yn[(sn + 1) % 3] + yn[(sn + 2) % 3]);
+                    // neighbour barycentre carried into this element's frame (periodic images)
+                    nbx[k] = (xn[0] + xn[1] + xn[2]) / 3 + (mx - nmx);
+                    nby[k] = (yn[0] + yn[1] + yn[2]) / 3 + (my - nmy);
+                } else {
+                    nbx[k] = 2 * mx - bx;
+                    nby[k] = 2 * my - by;
+                }
+            }
+            double l[LIM_ROWS];
+            int pk;
+            limiter_geometry(x, y, nbx, nby, l, &pk);
+            for (int r = 0; r < LIM_ROWS; ++r) hlim[(size_t)r * ld + p] = l[r];
+            hpairs[p] = pk;
+        }
+    }
+
+    // device
+    c->dev = prm->device;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    size_t ubytes = sizeof(double) * (size_t)4 * c->Np * ld;
+    for (int b = 0; b < 3; ++b) HIPCHK(c, hipMalloc(&c->U[b], ubytes));
+    HIPCHK(c, hipMalloc(&c->geo, sizeof(double) * hgeo.size()));
+    HIPCHK(c, hipMalloc(&c->nbr, sizeof(int) * hnbr.size()));
+    HIPCHK(c, hipMalloc(&c->dtflag, hflag.size()));
+    HIPCHK(c, hipMalloc(&c->flags, sizeof(int) * ld));
+    HIPCHK(c, hipMalloc(&c->st, sizeof(DevState)));
+    c->bytes = 3 * ubytes + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size() + sizeof(int) * ld;
+    for (int b = 0; b < 3; ++b) HIPCHK(c, hipMemsetAsync(c->U[b], 0, ubytes, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->flags, 0, sizeof(int) * ld, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->geo, hgeo.data(), sizeof(double) * hgeo.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->nbr, hnbr.data(), sizeof(int) * hnbr.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice, c->stream));
+    if (prm->limiter == RKDG_LIMITER_TVB) {
+        HIPCHK(c, hipMalloc(&c->lim, sizeof(double) * hlim.size()));
+        HIPCHK(c, hipMalloc(&c->pairs, sizeof(int) * hpairs.size()));
+        c->bytes += sizeof(double) * hlim.size() + sizeof(int) * hpairs.size();
+        HIPCHK(c, hipMemcpyAsync(c->lim, hlim.data(), sizeof(double) * hlim.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->pairs, hpairs.data(), sizeof(int) * hpairs.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    int rc = upload_tables(c->P, ref_basis(c->P).tab);
+    if (rc) { std::string m2 = hip_error_string(rc); rkdg_destroy(c); return fail(nullptr, RKDG_ERR_HIP, "table upload: " + m2); }
+    rc = launch_state_init(c->st, 0, 0.0, 1, c->stream);
+    if (rc) { rkdg_destroy(c); return fail(nullptr, RKDG_ERR_HIP, "state init"); }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = c;
+    return RKDG_OK;
+}
+
+int rkdg_destroy(rkdg_ctx* c) {
+    if (!c) return RKDG_OK;
+    hipSetDevice(c->dev);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph2) hipGraphExecDestroy(c->graph2);
+    for (int b = 0; b < 3; ++b) hipFree(c->U[b]);
+    hipFree(c->geo); hipFree(c->nbr); hipFree(c->dtflag); hipFree(c->lim); hipFree(c->pairs);
+    hipFree(c->flags); hipFree(c->st);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return RKDG_OK;
+}
+
+int rkdg_set_state(rkdg_ctx* c, int32_t basis, const double* U) {
+    if (!c || !U || (basis != RKDG_BASIS_ELEMENT && basis != RKDG_BASIS_NATIVE)) return fail(c, RKDG_ERR_ARG, "set_state");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const int E = c->E, Np = c->Np, P = c->P;
+    const long long ld = c->ld;
+    std::vector<double> h((size_t)4 * Np * ld, 0.0);
+    parallel_for(E, [&](int a, int b) {
+        double in[4 * MAXNP], outv[4 * MAXNP];
+        for (int p = a; p < b; ++p) {
+            int e = c->order[p];
+            for (int v = 0; v < 4; ++v)
+                for (int i = 0; i < Np; ++i) in[v * Np + i] = U[((size_t)v * Np + i) * E + e];
+            if (basis == RKDG_BASIS_ELEMENT) element_to_native(P, &c->ex[3 * e], &c->ey[3 * e], in, outv);
+            else std::memcpy(outv, in, sizeof(double) * 4 * Np);
+            for (int r = 0; r < 4 * Np; ++r) h[(size_t)r * ld + p] = outv[r];
+        }
+    });
+    HIPCHK(c, hipMemcpyAsync(c->U[0], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    c->cur = 0;
+    c->next_stage = 0;
+    c->steps = 0;
+    c->have_state = true;
+    int rc = refresh_wavespeed(c, 0, 0.0);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return check_physical(c);
+}
+
+int rkdg_get_state(rkdg_ctx* c, int32_t basis, double* U) {
+    if (!c || !U || (basis != RKDG_BASIS_ELEMENT && basis != RKDG_BASIS_NATIVE)) return fail(c, RKDG_ERR_ARG, "get_state");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const int E = c->E, Np = c->Np, P = c->P;
+    const long long ld = c->ld;
+    std::vector<double> h((size_t)4 * Np * ld);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->U[c->cur], sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    parallel_for(E, [&](int a, int b) {
+        double in[4 * MAXNP], outv[4 * MAXNP];
+        for (int p = a; p < b; ++p) {
+            int e = c->order[p];
+            for (int r = 0; r < 4 * Np; ++r) in[r] = h[(size_t)r * ld + p];
+            if (basis == RKDG_BASIS_ELEMENT) native_to_element(P, &c->ex[3 * e], &c->ey[3 * e], in, outv);
+            else std::memcpy(outv, in, sizeof(double) * 4 * Np);
+            for (int v = 0; v < 4; ++v)
+                for (int i = 0; i < Np; ++i) U[((size_t)v * Np + i) * E + e] = outv[v * Np + i];
+        }
+    });
+    return RKDG_OK;
+}
+
+int rkdg_project_ic(rkdg_ctx* c, int32_t kind, const double* p, int32_t np) {
+    if (!c || !p) return fail(c, RKDG_ERR_ARG, "project_ic");
+    int need = kind == RKDG_IC_FREESTREAM ? 5 : kind == RKDG_IC_VORTEX ? 6 : kind == RKDG_IC_RIEMANN4 ? 19 : -1;
+    if (need < 0 || np < need) return fail(c, RKDG_ERR_ARG, "initial condition kind or parameter count");
+    const int E = c->E, Np = c->Np, P = c->P;
+    std::vector<double> U((size_t)4 * Np * E, 0.0);
+    parallel_for(E, [&](int a, int b) {
+        double d[4 * MAXNP];
+        for (int e = a; e < b; ++e) {
+            if (kind == RKDG_IC_FREESTREAM) {
+                // mode 0 only, higher modes zero (src/cudarkdgsolver.cu:348-367)
+                double s[4];
+                ic_state(kind, p, np, 0, 0, s);
+                for (int r = 0; r < 4 * Np; ++r) d[r] = 0;
+                for (int v = 0; v < 4; ++v) d[v * Np] = s[v];
+            } else {
+                project_native(P, &c->ex[3 * e], &c->ey[3 * e], kind, p, np, d);
+            }
+            for (int v = 0; v < 4; ++v)
+                for (int i = 0; i < Np; ++i) U[((size_t)v * Np + i) * E + e] = d[v * Np + i];
+        }
+    });
+    return rkdg_set_state(c, RKDG_BASIS_NATIVE, U.data());
+}
+
+int rkdg_compute_dt(rkdg_ctx* c, double* dt) {
+    if (!c || !dt) return fail(c, RKDG_ERR_ARG, "compute_dt");
+    if (!c->have_state) return fail(c, RKDG_ERR_STATE, "no state");
+    if (c->prm.dt_policy == RKDG_DT_FIXED) { *dt = c->prm.dt_fixed; return RKDG_OK; }
+    HIPCHK(c, hipSetDevice(c->dev));
+    int slot = (int)(c->steps & 1);
+    DevState h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = refresh_wavespeed(c, slot, h.t[slot]);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double ws;
+    std::memcpy(&ws, &h.ws[slot], sizeof ws);
+    if (!(ws > 0)) return fail(c, RKDG_ERR_NONPHYSICAL, "no positive wave speed");
+    *dt = c->prm.cfl / ws;
+    return check_physical(c);
+}
+
+int rkdg_stage(rkdg_ctx* c, int32_t s, double dt) {
+    if (!c || s < 0 || s > 2 || !(dt >= 0)) return fail(c, RKDG_ERR_ARG, "stage");
+    if (!c->have_state) return fail(c, RKDG_ERR_STATE, "no state");
+    if (s != c->next_stage) return fail(c, RKDG_ERR_STATE, "stages must be called in order 0, 1, 2");
+    HIPCHK(c, hipSetDevice(c->dev));
+    int slot = (int)(c->steps & 1);
+    int rc = launch_one_stage(c, s, slot, DTM_HOST, dt, 0.0);
+    if (rc) return rc;
+    c->cur = s == 2 ? 0 : s + 1;
+    c->next_stage = (s + 1) % 3;
+    if (s == 2) c->steps++;
+    return RKDG_OK;
+}
+
+int rkdg_step(rkdg_ctx* c, double* dt_used) {
+    if (!c) return fail(c, RKDG_ERR_ARG, "step");
+    if (!c->have_state) return fail(c, RKDG_ERR_STATE, "no state");
+    if (c->next_stage != 0) return fail(c, RKDG_ERR_STATE, "a step started with rkdg_stage is unfinished");
+    HIPCHK(c, hipSetDevice(c->dev));
+    int slot = (int)(c->steps & 1);
+    int rc = launch_step(c, slot, 0.0);
+    if (rc) return rc;
+    c->steps++;
+    c->cur = 0;
+    if (dt_used) {
+        DevState h;
+        HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *dt_used = h.dt[slot];
+        if (h.err_pos != INT_MAX) return check_physical(c);
+    }
+    return RKDG_OK;
+}
+
+static int build_graph(rkdg_ctx* c) {
+    if (c->graph2) return RKDG_OK;
+    hipGraph_t g;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step(c, 0, 0.0);
+    if (!rc) rc = launch_step(c, 1, 0.0);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (rc) return rc;
+    if (e2 != hipSuccess) return fail(c, RKDG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e2));
+    hipError_t e3 = hipGraphInstantiate(&c->graph2, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e3 != hipSuccess) { c->graph2 = nullptr; return fail(c, RKDG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e3)); }
+    return RKDG_OK;
+}
+
+static int advance_impl(rkdg_ctx* c, int nsteps, double t_end, int* done_out, double* t_out) {
+    int use_graph = env_int("RKDG_GRAPH", 1) && t_end <= 0;
+    int done = 0;
+    if (t_end > 0) {
+        // chunks without host synchronisation inside; the end time turns the remaining
+        // launches of a chunk into no-ops on the device
+        const int chunk = 32;
+        DevState h;
+        while (done < nsteps) {
+            int n = std::min(chunk, nsteps - done);
+            for (int i = 0; i < n; ++i) {
+                int rc = launch_step(c, (int)(c->steps & 1), t_end);
+                if (rc) return rc;
+                c->steps++;
+            }
+            done += n;
+            HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (h.err_pos != INT_MAX) return check_physical(c);
+            if (h.t[c->steps & 1] >= t_end) break;
+        }
+    } else {
+        if (use_graph && (c->steps & 1)) {
+            int rc = launch_step(c, 1, 0.0);
+            if (rc) return rc;
+            c->steps++;
+            done++;
+        }
+        if (use_graph && nsteps - done >= 2) {
+            int rc = build_graph(c);
+            if (rc) return rc;
+            while (nsteps - done >= 2) {
+                HIPCHK(c, hipGraphLaunch(c->graph2, c->stream));
+                c->steps += 2;
+                done += 2;
+            }
+        }
+        while (done < nsteps) {
+            int rc = launch_step(c, (int)(c->steps & 1), 0.0);
+            if (rc) return rc;
+            c->steps++;
+            done++;
+        }
+    }
+    c->cur = 0;
+    if (done_out || t_out) {
+        DevState h;
+        HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (done_out) *done_out = (int)h.nsteps;
+        if (t_out) *t_out = h.t[c->steps & 1];
+        if (h.err_pos != INT_MAX) return check_physical(c);
+    }
+    (void)done;
+    return RKDG_OK;
+}
+
+int rkdg_advance(rkdg_ctx* c, int32_t nsteps, double t_end, int32_t* steps_done, double* t) {
+    if (!c || nsteps < 0) return fail(c, RKDG_ERR_ARG, "advance");
+    if (!c->have_state) return fail(c, RKDG_ERR_STATE, "no state");
+    if (c->next_stage != 0) return fail(c, RKDG_ERR_STATE, "a step started with rkdg_stage is unfinished");
+    HIPCHK(c, hipSetDevice(c->dev));
+    int d = 0;
+    int rc = advance_impl(c, nsteps, t_end, &d, t);
+    if (steps_done) *steps_done = d;
+    return rc;
+}
+
+int rkdg_residual(rkdg_ctx* c, double res[2]) {
+    if (!c || !res) return fail(c, RKDG_ERR_ARG, "residual");
+    HIPCHK(c, hipSetDevice(c->dev));
+    DevState h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(&res[0], &h.res[0], sizeof(double));
+    std::memcpy(&res[1], &h.res[1], sizeof(double));
+    return RKDG_OK;
+}
+
+int rkdg_time(rkdg_ctx* c, double* t, int64_t* steps) {
+    if (!c) return fail(c, RKDG_ERR_ARG, "time");
+    HIPCHK(c, hipSetDevice(c->dev));
+    DevState h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (t) *t = h.t[c->steps & 1];
+    if (steps) *steps = h.nsteps;
+    return RKDG_OK;
+}
+
+int rkdg_get_flags(rkdg_ctx* c, int32_t* flags) {
+    if (!c || !flags) return fail(c, RKDG_ERR_ARG, "get_flags");
+    HIPCHK(c, hipSetDevice(c->dev));
+    std::vector<int> h(c->ld);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->flags, sizeof(int) * c->ld, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int p = 0; p < c->E; ++p) flags[c->order[p]] = h[p];
+    return RKDG_OK;
+}
+
+int rkdg_sync(rkdg_ctx* c) {
+    if (!c) return fail(c, RKDG_ERR_ARG, "sync");
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RKDG_OK;
+}
+
+int rkdg_time_stage(rkdg_ctx* c, int32_t s, int32_t reps, double* avg_ms) {
+    if (!c || s < 0 || s > 1 || reps < 1 || !avg_ms) return fail(c, RKDG_ERR_ARG, "time_stage (stages 0 and 1 re-run idempotently)");
+    if (!c->have_state) return fail(c, RKDG_ERR_STATE, "no state");
+    HIPCHK(c, hipSetDevice(c->dev));
+    int slot = (int)(c->steps & 1);
+    double dt;
+    int rc = rkdg_compute_dt(c, &dt);
+    if (rc) return rc;
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    // warm-up
+    StageArgs args = make_args(c, s, slot, DTM_HOST, dt, 0.0);
+    launch_stage(c->P, c->prm.flux, c->prm.precision, c->block, args, c->stream);
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; ++r) {
+        int lrc = launch_stage(c->P, c->prm.flux, c->prm.precision, c->block, args, c->stream);
+        if (lrc) return fail(c, RKDG_ERR_HIP, hip_error_string(lrc));
+    }
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *avg_ms = ms / reps;
+    c->cur = 0;
+    return RKDG_OK;
+}
+
+int rkdg_time_advance(rkdg_ctx* c, int32_t nsteps, double* out) {
+    // out[0]: total ms over the step loop; out[1]: mean ms per stage kernel launch,
+    // bracketed individually; out[2]: stage kernel launches; out[3]: mean ms of a
+    // limiter launch (0 without limiter)
+    if (!c || nsteps < 1 || !out) return fail(c, RKDG_ERR_ARG, "time_advance");
+    if (!c->have_state) return fail(c, RKDG_ERR_STATE, "no state");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const int n = 3 * nsteps;
+    const bool lim = c->prm.limiter == RKDG_LIMITER_TVB;
+    std::vector<hipEvent_t> ev((size_t)2 * n + 2 + (lim ? 2 * n : 0));
+    for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+    int dt_mode = c->prm.dt_policy == RKDG_DT_FIXED ? DTM_FIXED : DTM_DEVICE;
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    for (int st = 0; st < nsteps; ++st) {
+        int slot = (int)(c->steps & 1);
+        for (int s = 0; s < 3; ++s) {
+            int k = st * 3 + s;
+            StageArgs a = make_args(c, s, slot, dt_mode, c->prm.dt_fixed, 0.0);
+            HIPCHK(c, hipEventRecord(ev[2 + 2 * k], c->stream));
+            int rc = launch_stage(c->P, c->prm.flux, c->prm.precision, c->block, a, c->stream);
+            if (rc) return fail(c, RKDG_ERR_HIP, hip_error_string(rc));
+            HIPCHK(c, hipEventRecord(ev[3 + 2 * k], c->stream));
+            if (lim) {
+                LimArgs la{};
+                la.U = a.Uout; la.lim = c->lim; la.pairs = c->pairs; la.nbr = c->nbr; la.flags = c->flags;
+                la.ld = c->ld; la.E = c->E; la.M = c->prm.tvb_M;
+                HIPCHK(c, hipEventRecord(ev[2 + 2 * n + 2 * k], c->stream));
+                rc = launch_limiter(c->P, la, c->block, c->stream);
+                if (rc) return fail(c, RKDG_ERR_HIP, hip_error_string(rc));
+                HIPCHK(c, hipEventRecord(ev[3 + 2 * n + 2 * k], c->stream));
+            }
+        }
+        c->steps++;
+    }
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    HIPCHK(c, hipEventSynchronize(ev[1]));
+    float ms;
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    out[0] = ms;
+    double sum = 0, suml = 0;
+    for (int k = 0; k < n; ++k) {
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[2 + 2 * k], ev[3 + 2 * k]));
+        sum += ms;
+        if (lim) {
+            HIPCHK(c, hipEventElapsedTime(&ms, ev[2 + 2 * n + 2 * k], ev[3 + 2 * n + 2 * k]));
+            suml += ms;
+        }
+    }
+    out[1] = sum / n;
+    out[2] = n;
+    out[3] = lim ? suml / n : 0.0;
+    for (auto& e : ev) hipEventDestroy(e);
+    c->cur = 0;
+    return check_physical(c);
+}
+
+int rkdg_error_vs_exact(rkdg_ctx* c, int32_t kind, const double* p, int32_t np, double out[8]) {
+    if (!c || !p || !out || (kind != RKDG_IC_VORTEX && kind != RKDG_IC_FREESTREAM)) return fail(c, RKDG_ERR_ARG, "error_vs_exact");
+    const int E = c->E, Np = c->Np, P = c->P;
+    std::vector<double> U((size_t)4 * Np * E);
+    int rc = rkdg_get_state(c, RKDG_BASIS_NATIVE, U.data());
+    if (rc) return rc;
+    const RefBasis& B = ref_basis(P);
+    const TriRule& R = projection_rule();
+    std::vector<double> psi((size_t)R.n * Np);
+    for (int q = 0; q < R.n; ++q)
+        for (int i = 0; i < Np; ++i) psi[(size_t)q * Np + i] = psi_value(B, i, R.xi[q], R.eta[q]);
+    const int nt = 16;
+    std::vector<double> acc((size_t)nt * 12, 0.0);
+    std::vector<int> chunk_id(1, 0);
+    std::atomic<int> next{0};
+    parallel_for(E, [&](int a, int b) {
+        int id = next++;
+        double* s = &acc[(size_t)(id % nt) * 12];
+        for (int e = a; e < b; ++e) {
+            const double* x = &c->ex[3 * e];
+            const double* y = &c->ey[3 * e];
+            double area = 0.5 * std::fabs((x[2] - x[0]) * (y[1] - y[0]) - (y[2] - y[0]) * (x[1] - x[0]));
+            for (int q = 0; q < R.n; ++q) {
+                double px = x[0] + (x[1] - x[0]) * R.xi[q] + (x[2] - x[0]) * R.eta[q];
+                double py = y[0] + (y[1] - y[0]) * R.xi[q] + (y[2] - y[0]) * R.eta[q];
+                double ex_[4];
+                ic_state(kind, p, np, px, py, ex_);
+                for (int v = 0; v < 4; ++v) {
+                    double u = 0;
+                    for (int i = 0; i < Np; ++i) u += U[((size_t)v * Np + i) * E + e] * psi[(size_t)q * Np + i];
+                    double d = u - ex_[v];
+                    s[v] += R.w[q] * area * d * d;
+                    s[4 + v] += R.w[q] * area * ex_[v] * ex_[v];
+                    s[8 + v] = std::max(s[8 + v], std::fabs(d));
+                }
+            }
+        }
+    });
+    double num[4] = {0, 0, 0, 0}, den[4] = {0, 0, 0, 0}, mx[4] = {0, 0, 0, 0};
+    for (int t = 0; t < nt; ++t)
+        for (int v = 0; v < 4; ++v) {
+            num[v] += acc[(size_t)t * 12 + v];
+            den[v] += acc[(size_t)t * 12 + 4 + v];
+            mx[v] = std::max(mx[v], acc[(size_t)t * 12 + 8 + v]);
+        }
+    for (int v = 0; v < 4; ++v) {
+        out[v] = den[v] > 0 ? std::sqrt(num[v] / den[v]) : std::sqrt(num[v]);
+        out[4 + v] = mx[v];
+    }
+    return RKDG_OK;
+}
+
+int rkdg_info(rkdg_ctx* c, int64_t out[5]) {
+    if (!c || !out) return RKDG_ERR_ARG;
+    out[0] = c->Np; out[1] = c->Nqv; out[2] = c->Nqe; out[3] = c->E; out[4] = (int64_t)c->bytes;
+    return RKDG_OK;
+}
+
+}  // extern "C"

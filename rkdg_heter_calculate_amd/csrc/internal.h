@@ -1,0 +1,151 @@
+// Internal declarations shared by the host C++ and the HIP kernels of librkdg.
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+#include "../../include/rkdg.h"
+
+namespace rkdg {
+
+constexpr int MAXNP = 10;   // P3
+constexpr int MAXQV = 12;
+constexpr int MAXQE = 4;
+constexpr int NVAR = 4;
+constexpr int GEO_ROWS = 14;  // xi_x, xi_y, eta_x, eta_y, area, l0, l1, l2, n0x, n0y, n1x, n1y, n2x, n2y
+constexpr int LIM_ROWS = 7;   // 3 x (a1, a2), h^2
+
+inline int np_of(int P) { return (P + 1) * (P + 2) / 2; }
+inline int nqv_of(int P) { return P <= 2 ? 7 : 12; }
+inline int nqe_of(int P) { return P <= 2 ? 3 : 4; }
+
+// Reference-element operators for one degree.  Reference triangle (0,0),(1,0),(0,1);
+// xi = lambda_1, eta = lambda_2.  The basis psi is orthonormal in the mean over the
+// triangle (psi_0 = 1), hierarchical by degree.
+struct DevTables {
+    double Vv[MAXQV][MAXNP];       // psi_i at the volume points
+    double Dxi[MAXQV][MAXNP];      // d psi_i / d xi
+    double Deta[MAXQV][MAXNP];     // d psi_i / d eta
+    double wv[MAXQV];              // volume weights as fractions of the area
+    double Ve[3][MAXQE][MAXNP];    // psi_i at edge k point j (edge k: vertex k+1 -> k+2)
+    double we[MAXQE];              // edge weights per half length
+    double minv[MAXNP];            // 1 / (quadrature mass per unit area)
+    double Vmid[3][3];             // P1 modes at the edge midpoints (limiter)
+    double recon[2][3];            // modes 1,2 from midpoint deviations (limiter)
+    // single-precision copies for the MFMA form
+    float fVv[16][16];             // [point (vol 0..Nqv-1, then 3*Nqe edge)][mode], zero padded
+    float fTest[16][16];           // [mode][point] test functions (gradients handled separately)
+};
+
+// Device-side step state: wave-speed maxima and times per step parity.
+struct DevState {
+    unsigned long long ws[2];      // max (|u|+c) P/A as double bits (non-negative)
+    double t[2];
+    double dt[2];
+    unsigned long long res[2];     // calculateResidual maxima (bits)
+    int err_pos;                   // smallest device position with non-physical mean (INT_MAX: none)
+    int pad;
+    long long nsteps;              // steps advanced (end-time no-ops excluded)
+};
+
+// Minimal fork-join helper for host loops (the library carries no OpenMP runtime).
+void parallel_for(int n, const std::function<void(int, int)>& body);
+
+// Host-side basis/geometry helpers (basis.cpp)
+struct RefBasis {
+    int P, Np, Nqv, Nqe;
+    long double C[MAXNP][MAXNP];   // psi_k = sum_m C[k][m] * mono_m(xi, eta)
+    DevTables tab;
+};
+const RefBasis& ref_basis(int P);
+double psi_value(const RefBasis& B, int i, double xi, double eta);
+void psi_grad(const RefBasis& B, int i, double xi, double eta, double* gx, double* ge);
+
+// collapsed 6x6 Gauss rule on the reference triangle; weights sum to 1
+struct TriRule { int n; std::vector<double> xi, eta, w; };
+const TriRule& projection_rule();
+
+// Element geometry rows for one element (GEO_ROWS doubles)
+void element_geometry(const double* x, const double* y, double* g);
+
+// The reference's per-element Gram-Schmidt basis (P <= 3), evaluated in physical
+// coordinates.  Built from monomials 1, X, Y, X^2, XY, Y^2, X^3, ... with
+// X = (x - xc)/sqrt(A), Y = (y - yc)/sqrt(A).
+struct ElemBasis {
+    int Np;
+    double xc, yc, isq;
+    double c[MAXNP][MAXNP];  // coefficients over the scaled monomials
+    double value(int i, double x, double y) const;
+};
+void element_basis(int P, const double* x, const double* y, ElemBasis& out);
+
+// Conversions for one element between coefficient sets (U: [4][Np] for this element)
+void element_to_native(int P, const double* x, const double* y, const double* c, double* d);
+void native_to_element(int P, const double* x, const double* y, const double* d, double* c);
+
+// Initial-condition states
+void ic_state(int kind, const double* p, int np, double x, double y, double* U);
+// L2 projection onto the native basis of one element
+void project_native(int P, const double* x, const double* y, int kind, const double* p, int np, double* d /*[4][Np]*/);
+
+// Limiter geometry of one element (LIM_ROWS doubles + packed pairs)
+void limiter_geometry(const double* x, const double* y, const double nbx[3], const double nby[3],
+                      double* lim, int* pairs);
+
+// Mesh helpers (grid.cpp)
+int build_shared_edges(int E, const int32_t* tri, const int32_t* nbr, int32_t* se, std::string& err);
+
+// Config parser (io.cpp)
+bool config_parse(const char* path, std::vector<std::pair<std::string, std::string>>& items);
+
+}  // namespace rkdg
+
+// ---- kernel launchers (kernels.hip) ---------------------------------------------------
+namespace rkdg {
+
+struct StageArgs {
+    const double* U;       // state the residual is evaluated on
+    const double* U0;      // step-start state (read by stages 1, 2)
+    double* Uout;          // stage result
+    const double* geo;     // [GEO_ROWS][ld]
+    const int* nbr;        // [3][ld] packed: (pos << 2) | shared edge, or -tag for a boundary
+    const unsigned char* dtflag;
+    DevState* st;
+    const int* tile_mesh;  // batching: mesh id per workgroup tile (NULL: single mesh)
+    DevState* st_batch;    // batching: per-mesh states
+    const double* cfl_batch;
+    long long ld;
+    int E;
+    int stage;
+    int slot;
+    int dt_mode;           // 0 device CFL, 1 host dt, 2 fixed
+    int xcd_remap;
+    int ntiles;            // workgroup tiles covering E
+    int persistent;        // >0: grid capped at this many workgroups, each looping over tiles
+    double dt_host;
+    double cfl, gamma, t_end;
+    double fs[4];
+};
+
+enum { DTM_DEVICE = 0, DTM_HOST = 1, DTM_FIXED = 2 };
+
+struct LimArgs {
+    double* U;
+    const double* lim;     // [LIM_ROWS][ld]
+    const int* pairs;      // [ld] packed neighbour pairs
+    const int* nbr;
+    int* flags;
+    long long ld;
+    int E;
+    double M;
+};
+
+int upload_tables(int P, const DevTables& t);  // per current device
+int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream);
+int launch_wavespeed(int P, const StageArgs& a, int block, void* stream);
+int launch_state_init(DevState* st, int slot, double t, int n, void* stream);
+int launch_limiter(int P, const LimArgs& a, int block, void* stream);
+const char* hip_error_string(int code);
+
+}  // namespace rkdg

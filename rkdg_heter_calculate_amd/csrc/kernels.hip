@@ -1,0 +1,577 @@
+// gfx950 kernels of the plain FP64 form.
+//
+// One fused kernel per SSP-RK3 stage replaces the reference's nine launches per stage
+// (calculateConVars, boundaryCondition, calculateVolumeRHS, calculateLFCoeff,
+// calculateEdgeFG, calculateFlux x2, calculateEdgeRHS, rkdgStep*;
+// src/kernelfunctions.cu:84-748).  One lane owns one element:
+//   * its modal state is read once, coalesced, from the SoA layout U[var][mode][ld];
+//   * the volume integral uses reference-element operators held in constant memory
+//     (uniform across the wavefront, so they are scalar loads) and the element's
+//     inverse Jacobian;
+//   * each edge's flux is computed by the element itself from its own trace and the
+//     neighbour's trace, evaluated from the neighbour's modes at the mirrored point
+//     (element-owned gather: no atomics, no scatter, no trace buffers in HBM);
+//   * the SSP-RK3 combination and the division by the diagonal mass are applied before
+//     the single store;
+//   * the last stage also reduces the CFL wave speed of the new cell means (the next
+//     step's dt) and the reference's residual, so dt never leaves the device.
+// Workgroups are mapped XCD-contiguously: with round-robin dispatch over the 8 XCDs,
+// workgroup b runs on XCD b%8, and it processes tile (b%8)*(tiles/8) + b/8 so that each
+// XCD sweeps one contiguous range of elements and neighbour gathers hit its own L2.
+#include <hip/hip_runtime.h>
+#include <climits>
+#include "internal.h"
+
+namespace rkdg {
+
+__constant__ DevTables c_tab[3];
+
+template <int P>
+struct Deg {
+    static constexpr int Np = (P + 1) * (P + 2) / 2;
+    static constexpr int Nqv = P <= 2 ? 7 : 12;
+    static constexpr int Nqe = P <= 2 ? 3 : 4;
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide max of three non-negative values, then one atomic per value.
+__device__ __forceinline__ void block_max3_atomic(double a, double b, double c, unsigned long long* pa,
+                                                  unsigned long long* pb, unsigned long long* pc, bool do_bc) {
+    __shared__ double red[3][16];
+    a = wave_max(a);
+    if (do_bc) { b = wave_max(b); c = wave_max(c); }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if (lane == 0) { red[0][wid] = a; red[1][wid] = b; red[2][wid] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ma = 0, mb = 0, mc = 0;
+        for (int w = 0; w < nw; ++w) { ma = fmax(ma, red[0][w]); mb = fmax(mb, red[1][w]); mc = fmax(mc, red[2][w]); }
+        if (ma > 0) atomicMax(pa, (unsigned long long)__double_as_longlong(ma));
+        if (do_bc) {
+            if (mb > 0) atomicMax(pb, (unsigned long long)__double_as_longlong(mb));
+            if (mc > 0) atomicMax(pc, (unsigned long long)__double_as_longlong(mc));
+        }
+    }
+}
+
+// Physical fluxes F, G of the Euler equations (kernel_calculateEdgeFG's formulas,
+// src/kernelfunctions.cu:413-428, with rho*u taken directly from the state).
+__device__ __forceinline__ void euler_fg(double g, const double* U, double* F, double* G) {
+    const double ir = 1.0 / U[0];
+    const double u = U[1] * ir, v = U[2] * ir;
+    const double p = (g - 1.0) * (U[3] - 0.5 * U[0] * (u * u + v * v));
+    F[0] = U[1]; F[1] = U[1] * u + p; F[2] = U[1] * v; F[3] = u * (U[3] + p);
+    G[0] = U[2]; G[1] = U[2] * u; G[2] = U[2] * v + p; G[3] = v * (U[3] + p);
+}
+
+// Roe (1981) normal flux with the Harten-Hyman (1983) entropy fix on the acoustic waves.
+__device__ __forceinline__ void roe_flux(double g, const double* UL, const double* UR, double nx, double ny,
+                                         double* Fh) {
+    double FL[4], GL[4], FR[4], GR[4];
+    euler_fg(g, UL, FL, GL);
+    euler_fg(g, UR, FR, GR);
+    const double rl = UL[0], ul = UL[1] / rl, vl = UL[2] / rl;
+    const double pl = (g - 1) * (UL[3] - 0.5 * rl * (ul * ul + vl * vl));
+    const double rr = UR[0], ur = UR[1] / rr, vr = UR[2] / rr;
+    const double pr = (g - 1) * (UR[3] - 0.5 * rr * (ur * ur + vr * vr));
+    const double hl = (UL[3] + pl) / rl, hr = (UR[3] + pr) / rr;
+    const double sl = sqrt(rl), sr = sqrt(rr), isum = 1.0 / (sl + sr);
+    const double rt = sl * sr;
+    const double ut = (sl * ul + sr * ur) * isum, vt = (sl * vl + sr * vr) * isum, ht = (sl * hl + sr * hr) * isum;
+    const double q2 = ut * ut + vt * vt;
+    const double c2 = (g - 1) * (ht - 0.5 * q2);
+    const double ct = sqrt(c2);
+    const double unt = ut * nx + vt * ny, utt = -ut * ny + vt * nx;
+    const double dp = pr - pl, dr = rr - rl;
+    const double dun = (ur * nx + vr * ny) - (ul * nx + vl * ny);
+    const double dut = (-ur * ny + vr * nx) - (-ul * ny + vl * nx);
+    const double a1 = (dp - rt * ct * dun) / (2 * c2);
+    const double a2 = dr - dp / c2;
+    const double a3 = rt * dut;
+    const double a4 = (dp + rt * ct * dun) / (2 * c2);
+    // Harten-Hyman entropy fix on the acoustic waves (vanishes across shocks)
+    const double cl = sqrt(g * pl / rl), cr = sqrt(g * pr / rr);
+    const double unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
+    double l1 = fabs(unt - ct), l2 = fabs(unt), l4 = fabs(unt + ct);
+    const double d1 = fmax(0.0, fmax((unt - ct) - (unl - cl), (unr - cr) - (unt - ct)));
+    const double d4 = fmax(0.0, fmax((unt + ct) - (unl + cl), (unr + cr) - (unt + ct)));
+    if (l1 < d1) l1 = (l1 * l1 + d1 * d1) / (2 * d1);
+    if (l4 < d4) l4 = (l4 * l4 + d4 * d4) / (2 * d4);
+    double d[4];
+    d[0] = l1 * a1 + l2 * a2 + l4 * a4;
+    d[1] = l1 * a1 * (ut - ct * nx) + l2 * a2 * ut + l2 * a3 * (-ny) + l4 * a4 * (ut + ct * nx);
+    d[2] = l1 * a1 * (vt - ct * ny) + l2 * a2 * vt + l2 * a3 * nx + l4 * a4 * (vt + ct * ny);
+    d[3] = l1 * a1 * (ht - unt * ct) + l2 * a2 * 0.5 * q2 + l2 * a3 * utt + l4 * a4 * (ht + unt * ct);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fh[v] = 0.5 * ((FL[v] + FR[v]) * nx + (GL[v] + GR[v]) * ny - d[v]);
+}
+
+struct TileInfo {
+    int tile;
+    int mesh;
+    bool first;
+    DevState* st;
+    double cfl;
+};
+
+// Workgroup iteration `it` -> tile.  Grids are multiples of 8 and a persistent
+// workgroup advances `it` by the grid size, so it % 8 (its XCD under round-robin
+// dispatch) never changes: each XCD sweeps one contiguous block of tiles.
+__device__ __forceinline__ bool tile_of_iter(const StageArgs& a, int it, int nit, TileInfo& ti) {
+    int tile = it;
+    if (a.xcd_remap) {
+        const int per = nit >> 3;  // nit is a multiple of 8
+        tile = (it & 7) * per + (it >> 3);
+    }
+    if (tile >= a.ntiles) return false;
+    ti.tile = tile;
+    if (a.tile_mesh) {
+        const int tm = a.tile_mesh[tile];
+        ti.mesh = tm >> 1;
+        ti.first = tm & 1;
+        ti.st = a.st_batch + ti.mesh;
+        ti.cfl = a.cfl_batch[ti.mesh];
+    } else {
+        ti.mesh = 0;
+        ti.first = tile == 0;
+        ti.st = a.st;
+        ti.cfl = a.cfl;
+    }
+    return true;
+}
+
+__device__ __forceinline__ int iter_count(const StageArgs& a) {
+    return a.xcd_remap ? (a.ntiles + 7) / 8 * 8 : a.ntiles;
+}
+
+// dt of the current step, identical in every workgroup of the mesh; done = end time reached
+__device__ __forceinline__ double stage_dt(const StageArgs& a, const TileInfo& ti, bool& done) {
+    double dt;
+    if (a.dt_mode == DTM_DEVICE) dt = ti.cfl / __longlong_as_double((long long)ti.st->ws[a.slot]);
+    else dt = a.dt_host;
+    done = false;
+    if (a.t_end > 0) {
+        const double rem = a.t_end - ti.st->t[a.slot];
+        if (!(rem > 0)) done = true;
+        else if (rem < dt) dt = rem;
+    }
+    return dt;
+}
+
+template <int P, int FLUX>
+__device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& ti, const double* sVe) {
+    using D = Deg<P>;
+    constexpr int Np = D::Np, Nqv = D::Nqv, Nqe = D::Nqe;
+    const DevTables& T = c_tab[P - 1];
+    bool done;
+    const double dt = stage_dt(a, ti, done);
+    if (ti.first && threadIdx.x == 0) {
+        if (a.stage == 0 && !done) {
+            ti.st->ws[1 - a.slot] = 0ull;
+            ti.st->res[0] = 0ull;
+            ti.st->res[1] = 0ull;
+            ti.st->dt[a.slot] = dt;
+        }
+        if (a.stage == 2) {
+            ti.st->t[1 - a.slot] = done ? ti.st->t[a.slot] : ti.st->t[a.slot] + dt;
+            if (done) ti.st->ws[1 - a.slot] = ti.st->ws[a.slot];
+            else ti.st->nsteps += 1;
+        }
+    }
+    if (done) return;
+
+    const long long ld = a.ld;
+    const int e = ti.tile * blockDim.x + threadIdx.x;
+    const double g = a.gamma;
+    double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
+
+    if (e < a.E) {
+        // geometry: inverse Jacobian, area, edge lengths, outward normals
+        double geo[GEO_ROWS];
+#pragma unroll
+        for (int r = 0; r < GEO_ROWS; ++r) geo[r] = a.geo[r * ld + e];
+        const double xix = geo[0], xiy = geo[1], etx = geo[2], ety = geo[3], area = geo[4];
+
+        double Uo[4][Np], R[4][Np];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 0; i < Np; ++i) {
+                Uo[v][i] = a.U[(v * Np + i) * ld + e];
+                R[v][i] = 0.0;
+            }
+
+        // ---- volume integral: sum_q w_q A (F . grad psi_i) -------------------------
+#pragma unroll
+        for (int q = 0; q < Nqv; ++q) {
+            double uq[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < Np; ++i) s += Uo[v][i] * T.Vv[q][i];
+                uq[v] = s;
+            }
+            double F[4], G[4];
+            euler_fg(g, uq, F, G);
+            const double wa = T.wv[q] * area;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const double fc = wa * (F[v] * xix + G[v] * xiy);
+                const double gc = wa * (F[v] * etx + G[v] * ety);
+#pragma unroll
+                for (int i = 1; i < Np; ++i) R[v][i] += fc * T.Dxi[q][i] + gc * T.Deta[q][i];
+            }
+        }
+
+        // ---- surface integral, element-owned -------------------------------------
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int w = a.nbr[k * ld + e];
+            const double lk = geo[5 + k], nx = geo[8 + 2 * k], ny = geo[9 + 2 * k];
+            double uL[Nqe][4], uR[Nqe][4], mR[4];
+#pragma unroll
+            for (int j = 0; j < Nqe; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int i = 0; i < Np; ++i) s += Uo[v][i] * T.Ve[k][j][i];
+                    uL[j][v] = s;
+                }
+            if (w >= 0) {
+                const int n = w >> 2, sn = w & 3;
+                const double* tb = sVe + sn * (Nqe * Np);
+#pragma unroll
+                for (int j = 0; j < Nqe; ++j)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) uR[j][v] = 0.0;
+#pragma unroll
+                for (int i = 0; i < Np; ++i) {
+                    double c[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) c[v] = a.U[(v * Np + i) * ld + n];
+                    if (i == 0) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) mR[v] = c[v];
+                    }
+#pragma unroll
+                    for (int j = 0; j < Nqe; ++j) {
+                        const double t = tb[(Nqe - 1 - j) * Np + i];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) uR[j][v] += c[v] * t;
+                    }
+                }
+            } else if (w == -RKDG_BC_FARFIELD) {
+                // kernel_boundaryCondition CELL_FARFIELD (src/kernelfunctions.cu:184-201)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    mR[v] = a.fs[v];
+#pragma unroll
+                    for (int j = 0; j < Nqe; ++j) uR[j][v] = a.fs[v];
+                }
+            } else if (w == -RKDG_BC_REFLECTION) {
+                // CELL_REFLECTION (232-268): velocity mirrored about the edge normal
+                const double cxx = ny * ny - nx * nx, cyy = nx * nx - ny * ny, cxy = 2.0 * nx * ny;
+#pragma unroll
+                for (int j = 0; j < Nqe; ++j) {
+                    uR[j][0] = uL[j][0];
+                    uR[j][3] = uL[j][3];
+                    uR[j][1] = uL[j][1] * cxx - cxy * uL[j][2];
+                    uR[j][2] = uL[j][2] * cyy - cxy * uL[j][1];
+                }
+                mR[0] = Uo[0][0];
+                mR[3] = Uo[3][0];
+                mR[1] = Uo[1][0] * cxx - cxy * Uo[2][0];
+                mR[2] = Uo[2][0] * cyy - cxy * Uo[1][0];
+            } else {
+                // CELL_OUTFLOW (203-230): transmissive
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    mR[v] = Uo[v][0];
+#pragma unroll
+                    for (int j = 0; j < Nqe; ++j) uR[j][v] = uL[j][v];
+                }
+            }
+
+            double alpha = 0.0;
+            if (FLUX == RKDG_FLUX_LF) {
+                // kernel_calculateLFCoeff (src/kernelfunctions.cu:370-387)
+                const double rho = 0.5 * (Uo[0][0] + mR[0]);
+                const double u = 0.5 * (Uo[1][0] + mR[1]) / rho;
+                const double v = 0.5 * (Uo[2][0] + mR[2]) / rho;
+                const double rE = 0.5 * (Uo[3][0] + mR[3]);
+                const double p = (g - 1.0) * (rE - 0.5 * rho * (u * u + v * v));
+                alpha = fabs(u * nx + v * ny) + sqrt(g * p / rho);
+            }
+#pragma unroll
+            for (int j = 0; j < Nqe; ++j) {
+                double Fh[4];
+                if (FLUX == RKDG_FLUX_LF) {
+                    double FL[4], GL[4], FR[4], GR[4];
+                    euler_fg(g, uL[j], FL, GL);
+                    euler_fg(g, uR[j], FR, GR);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        Fh[v] = 0.5 * ((FL[v] + FR[v]) * nx + (GL[v] + GR[v]) * ny - alpha * (uR[j][v] - uL[j][v]));
+                } else {
+                    roe_flux(g, uL[j], uR[j], nx, ny, Fh);
+                }
+                const double wj = T.we[j] * 0.5 * lk;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const double f = wj * Fh[v];
+#pragma unroll
+                    for (int i = 0; i < Np; ++i) R[v][i] -= f * T.Ve[k][j][i];
+                }
+            }
+        }
+
+        // ---- SSP-RK3 stage (src/kernelfunctions.cu:640-748) -----------------------
+        const double ia = 1.0 / area;
+        double mean[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 0; i < Np; ++i) {
+                const long long idx = (v * Np + i) * ld + e;
+                const double r = R[v][i] * (T.minv[i] * ia);
+                double out;
+                if (a.stage == 0) {
+                    out = Uo[v][i] + dt * r;
+                } else if (a.stage == 1) {
+                    out = 0.75 * a.U0[idx] + 0.25 * Uo[v][i] + 0.25 * dt * r;
+                } else {
+                    if (i == 0) {
+                        const double o0 = a.U0[idx];
+                        out = (1.0 / 3.0) * o0 + (2.0 / 3.0) * Uo[v][i] + (2.0 / 3.0) * dt * r;
+                        if (a.dtflag[e]) {
+                            if (v == 0) r0_l = fabs(out - o0);
+                            if (v == 3) r1_l = fabs(out - o0);
+                        }
+                    } else {
+                        out = (1.0 / 3.0) * a.U0[idx] + (2.0 / 3.0) * Uo[v][i] + (2.0 / 3.0) * dt * r;
+                    }
+                }
+                if (i == 0) mean[v] = out;
+                a.Uout[idx] = out;
+            }
+
+        if (a.stage == 2) {
+            // next step's CFL wave speed on the new means (kernel_getTimeStep's formula,
+            // src/kernelfunctions.cu:19-25)
+            const double rho = mean[0], u = mean[1] / rho, v = mean[2] / rho;
+            const double p = (g - 1.0) * (mean[3] - 0.5 * rho * (u * u + v * v));
+            if (!(rho > 0.0) || !(p > 0.0)) atomicMin(&ti.st->err_pos, e);
+            const double c = sqrt(g * p / rho);
+            const double s = (sqrt(u * u + v * v) + c) * (geo[5] + geo[6] + geo[7]) / area;
+            if (a.dtflag[e] && s > 0.0) ws_l = s;
+        }
+    }
+    if (a.stage == 2)
+        block_max3_atomic(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], &ti.st->res[0], &ti.st->res[1], true);
+}
+
+template <int P, int FLUX>
+__global__ void __launch_bounds__(256) k_stage(StageArgs a) {
+    using D = Deg<P>;
+    constexpr int Np = D::Np, Nqe = D::Nqe;
+    const DevTables& T = c_tab[P - 1];
+    __shared__ double sVe[3 * Nqe * Np];
+    for (int t = threadIdx.x; t < 3 * Nqe * Np; t += blockDim.x) {
+        const int k = t / (Nqe * Np), r = t - k * (Nqe * Np), j = r / Np, i = r - j * Np;
+        sVe[t] = T.Ve[k][j][i];
+    }
+    __syncthreads();
+    const int nit = iter_count(a);
+    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
+        TileInfo ti;
+        if (!tile_of_iter(a, it, nit, ti)) continue;  // uniform across the workgroup
+        stage_tile<P, FLUX>(a, ti, sVe);
+        __syncthreads();  // the reduction scratch is reused by the next tile
+    }
+}
+
+// CFL wave speed of the current state, max-reduced into ws[slot]
+template <int P>
+__global__ void __launch_bounds__(256) k_wavespeed(StageArgs a) {
+    constexpr int Np = Deg<P>::Np;
+    const long long ld = a.ld;
+    const int nit = iter_count(a);
+    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
+        TileInfo ti;
+        if (!tile_of_iter(a, it, nit, ti)) continue;
+        const int e = ti.tile * blockDim.x + threadIdx.x;
+        double s_l = 0.0;
+        if (e < a.E) {
+            const double g = a.gamma;
+            const double rho = a.U[(0 * Np) * ld + e];
+            const double u = a.U[(1 * Np) * ld + e] / rho, v = a.U[(2 * Np) * ld + e] / rho;
+            const double p = (g - 1.0) * (a.U[(3 * Np) * ld + e] - 0.5 * rho * (u * u + v * v));
+            if (!(rho > 0.0) || !(p > 0.0)) atomicMin(&ti.st->err_pos, e);
+            const double c = sqrt(g * p / rho);
+            const double area = a.geo[4 * ld + e];
+            const double s = (sqrt(u * u + v * v) + c) * (a.geo[5 * ld + e] + a.geo[6 * ld + e] + a.geo[7 * ld + e]) / area;
+            if (a.dtflag[e] && s > 0.0) s_l = s;
+        }
+        block_max3_atomic(s_l, 0.0, 0.0, &ti.st->ws[a.slot], nullptr, nullptr, false);
+        __syncthreads();
+    }
+}
+
+__global__ void k_state_init(DevState* st, int n, int slot, double t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        st[i].ws[0] = 0ull; st[i].ws[1] = 0ull;
+        st[i].t[slot] = t; st[i].t[1 - slot] = t;
+        st[i].dt[0] = 0.0; st[i].dt[1] = 0.0;
+        st[i].res[0] = 0ull; st[i].res[1] = 0ull;
+        st[i].err_pos = INT_MAX;
+        st[i].nsteps = 0;
+    }
+}
+
+// TVB minmod limiter (Cockburn & Shu 1998) on the P1 part, component-wise, in place.
+// Reads the neighbours' means only; the means are never modified, so the in-place
+// update is race-free.
+template <int P>
+__global__ void __launch_bounds__(256) k_limit(LimArgs a) {
+    constexpr int Np = Deg<P>::Np;
+    const DevTables& T = c_tab[P - 1];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.E) return;
+    const long long ld = a.ld;
+    double m0[4], s1[4], s2[4], nm[3][4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        m0[v] = a.U[(v * Np + 0) * ld + e];
+        s1[v] = a.U[(v * Np + 1) * ld + e];
+        s2[v] = a.U[(v * Np + 2) * ld + e];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int w = a.nbr[k * ld + e];
+        if (w >= 0) {
+            const int n = w >> 2;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) nm[k][v] = a.U[(v * Np) * ld + n];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) nm[k][v] = m0[v];
+        }
+    }
+    double al[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) al[r] = a.lim[r * ld + e];
+    const double tol = a.M * a.lim[6 * ld + e];
+    const int pk = a.pairs[e];
+    const double nu = 1.5;
+    int troubled = 0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        double d[3];
+        bool changed = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double ut = T.Vmid[k][1] * s1[v] + T.Vmid[k][2] * s2[v];
+            const int k1 = (pk >> (4 * k)) & 3, k2 = (pk >> (4 * k + 2)) & 3;
+            double n1 = nm[0][v], n2 = nm[0][v];
+            if (k1 == 1) n1 = nm[1][v]; else if (k1 == 2) n1 = nm[2][v];
+            if (k2 == 1) n2 = nm[1][v]; else if (k2 == 2) n2 = nm[2][v];
+            const double b = nu * (al[2 * k] * (n1 - m0[v]) + al[2 * k + 1] * (n2 - m0[v]));
+            double r;
+            if (fabs(ut) <= tol) r = ut;
+            else if (ut > 0 && b > 0) r = fmin(ut, b);
+            else if (ut < 0 && b < 0) r = fmax(ut, b);
+            else r = 0.0;
+            if (r != ut) changed = true;
+            d[k] = r;
+        }
+        if (!changed) continue;
+        troubled = 1;
+        const double sum = d[0] + d[1] + d[2];
+        if (sum != 0.0) {
+            double pos = 0, neg = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { pos += fmax(0.0, d[k]); neg += fmax(0.0, -d[k]); }
+            const double tp = pos > 0 ? fmin(1.0, neg / pos) : 0.0;
+            const double tn = neg > 0 ? fmin(1.0, pos / neg) : 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) d[k] = tp * fmax(0.0, d[k]) - tn * fmax(0.0, -d[k]);
+        }
+        a.U[(v * Np + 1) * ld + e] = T.recon[0][0] * d[0] + T.recon[0][1] * d[1] + T.recon[0][2] * d[2];
+        a.U[(v * Np + 2) * ld + e] = T.recon[1][0] * d[0] + T.recon[1][1] * d[1] + T.recon[1][2] * d[2];
+#pragma unroll
+        for (int i = 3; i < Np; ++i) a.U[(v * Np + i) * ld + e] = 0.0;
+    }
+    a.flags[e] = troubled;
+}
+
+int upload_tables(int P, const DevTables& t) {
+    hipError_t rc = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof(DevTables), (size_t)(P - 1) * sizeof(DevTables),
+                                      hipMemcpyHostToDevice);
+    return (int)rc;
+}
+
+static int grid_blocks(const StageArgs& a) {
+    int nb = a.xcd_remap ? (a.ntiles + 7) / 8 * 8 : a.ntiles;
+    if (a.persistent > 0 && a.persistent < nb) nb = a.xcd_remap ? (a.persistent + 7) / 8 * 8 : a.persistent;
+    return nb;
+}
+
+template <int P>
+static void launch_stage_p(int flux, int block, const StageArgs& a, hipStream_t s) {
+    dim3 grid(grid_blocks(a)), blk(block);
+    if (flux == RKDG_FLUX_ROE) hipLaunchKernelGGL((k_stage<P, RKDG_FLUX_ROE>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_stage<P, RKDG_FLUX_LF>), grid, blk, 0, s, a);
+}
+
+int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream) {
+    (void)prec;
+    hipStream_t s = (hipStream_t)stream;
+    switch (P) {
+        case 1: launch_stage_p<1>(flux, block, a, s); break;
+        case 2: launch_stage_p<2>(flux, block, a, s); break;
+        case 3: launch_stage_p<3>(flux, block, a, s); break;
+        default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_wavespeed(int P, const StageArgs& a, int block, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid(grid_blocks(a)), blk(block);
+    switch (P) {
+        case 1: hipLaunchKernelGGL(k_wavespeed<1>, grid, blk, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_wavespeed<2>, grid, blk, 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_wavespeed<3>, grid, blk, 0, s, a); break;
+        default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_state_init(DevState* st, int slot, double t, int n, void* stream) {
+    hipLaunchKernelGGL(k_state_init, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, st, n, slot, t);
+    return (int)hipGetLastError();
+}
+
+int launch_limiter(int P, const LimArgs& a, int block, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid((a.E + block - 1) / block), blk(block);
+    switch (P) {
+        case 1: hipLaunchKernelGGL(k_limit<1>, grid, blk, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_limit<2>, grid, blk, 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_limit<3>, grid, blk, 0, s, a); break;
+        default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+const char* hip_error_string(int code) { return hipGetErrorString((hipError_t)code); }
+
+}  // namespace rkdg

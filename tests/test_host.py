@@ -1,0 +1,238 @@
+"""Host logic of librkdg without a GPU: exported symbols, mesh generation and ingest,
+config parser, .plt writer, state dump, basis exchange (CPU)."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from rkdg_heter_calculate_amd import Grid, _lib as L
+from rkdg_heter_calculate_amd.solver import freestream_state
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def test_library_exports_every_declared_symbol():
+    with open(os.path.join(ROOT, "include", "rkdg.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(rkdg_[a-z0-9_]+)\s*\(", text))
+    lib = ctypes.CDLL(L.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert len(names) >= 40
+    # and the Python binding knows every one of them
+    bound = {n for n, _, _ in L.SIGNATURES}
+    assert names <= bound, names - bound
+
+
+def check_connectivity(g, periodic, lx=None, ly=None):
+    E = g.n_elements
+    ex, ey = g.element_coords()
+    cross = (ex[:, 1] - ex[:, 0]) * (ey[:, 2] - ey[:, 0]) - (ey[:, 1] - ey[:, 0]) * (ex[:, 2] - ex[:, 0])
+    assert (cross > 0).all()  # CCW
+    for e in range(E):
+        for k in range(3):
+            n = g.neighbour[e, k]
+            if n < 0:
+                assert not periodic
+                continue
+            s = g.shared_edge[e, k]
+            assert g.neighbour[n, s] == e and g.shared_edge[n, s] == k
+            # same physical edge (up to a period), opposite direction
+            a = np.array([ex[e, (k + 1) % 3], ey[e, (k + 1) % 3]])
+            b = np.array([ex[e, (k + 2) % 3], ey[e, (k + 2) % 3]])
+            c = np.array([ex[n, (s + 1) % 3], ey[n, (s + 1) % 3]])
+            d = np.array([ex[n, (s + 2) % 3], ey[n, (s + 2) % 3]])
+            sh = (a - d)
+            if periodic:
+                sh = sh - np.round(sh / [lx, ly]) * [lx, ly]
+            assert np.allclose(sh, 0, atol=1e-12)
+            assert np.allclose((a - d) - (b - c), 0, atol=1e-12)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_splitquad_connectivity(periodic):
+    g = Grid.splitquad(9, 7, 0.0, 0.0, 10.0, 7.0, periodic=periodic, diagonal=1, jitter=0.2, seed=3)
+    assert g.n_elements == 2 * 9 * 7
+    check_connectivity(g, periodic, 10.0, 7.0)
+    if not periodic:
+        nb = (g.neighbour < 0).sum()
+        assert nb == 2 * (9 + 7)
+        assert (g.boundary[g.neighbour < 0] == L.BC_OUTFLOW).all()
+
+
+def test_splitquad_jitter_and_seed():
+    n = 16
+    a = Grid.splitquad(n, n, 0.0, 0.0, 10.0, 10.0, periodic=True, diagonal=1, jitter=0.2, seed=42)
+    b = Grid.splitquad(n, n, 0.0, 0.0, 10.0, 10.0, periodic=True, diagonal=1, jitter=0.2, seed=42)
+    c = Grid.splitquad(n, n, 0.0, 0.0, 10.0, 10.0, periodic=True, diagonal=1, jitter=0.2, seed=43)
+    assert np.array_equal(a.vx, b.vx) and np.array_equal(a.tri, b.tri)
+    assert not np.array_equal(a.vx, c.vx)
+    h = 10.0 / n
+    i = np.arange((n + 1) * (n + 1)) % (n + 1)
+    j = np.arange((n + 1) * (n + 1)) // (n + 1)
+    dx, dy = a.vx - i * h, a.vy - j * h
+    assert np.abs(dx).max() <= 0.2 * h and np.abs(dy).max() <= 0.2 * h
+    bnd = (i == 0) | (i == n) | (j == 0) | (j == n)
+    assert np.all(dx[bnd] == 0) and np.all(dy[bnd] == 0)
+    # both diagonals occur
+    d0 = Grid.splitquad(n, n, 0.0, 0.0, 10.0, 10.0, periodic=True, diagonal=0)
+    assert not np.array_equal(a.tri, d0.tri)
+
+
+def write_reference_mesh(tmp_path, d, with_header_neigh=True):
+    """Reconstructs the reference mesh files from the fixture (headers as the originals)."""
+    inp = tmp_path / "input"
+    inp.mkdir()
+    E, NV = d["tri"].shape[0], d["vertices"].shape[0]
+    (inp / "Vertice.dat").write_text("#vertist data\n" + "\n".join(f"{float(x)!r} {float(y)!r}" for x, y in d["vertices"]) + "\n")
+    (inp / "ElemVertice.dat").write_text("#ElemVertex Data\n" + "\n".join(" ".join(map(str, r)) for r in d["tri"]) + "\n")
+    (inp / "ElemNeigh.dat").write_text(("#ElemNeigh data\n" if with_header_neigh else "") +
+                                      "\n".join(" ".join(map(str, r)) for r in d["neigh"]) + "\n")
+    (inp / "ElemEdge.dat").write_text("#Elemside data\n" + "\n".join(" ".join(map(str, r)) for r in d["edge"]) + "\n")
+    (inp / "EdgeVertice.dat").write_text("#Sidelist data\n" + "\n".join(" ".join(map(str, r)) for r in d["edge_vertices"]) + "\n")
+    (inp / "ElemDepart.dat").write_text("\n".join(map(str, d["partition"])) + "\n")
+    conf = tmp_path / "mesh.conf"
+    conf.write_text(f"trianglenum {E}\nverticenum {NV}\nedgenum {d['edge_vertices'].shape[0]}\n"
+                    f"trivertice_filename {inp}/ElemVertice.dat\ntrineigh_filename {inp}/ElemNeigh.dat\n"
+                    f"triedge_filename {inp}/ElemEdge.dat\nvertice_filename {inp}/Vertice.dat\n"
+                    f"edgevertice_filename {inp}/EdgeVertice.dat\noutput_filename out.plt\n")
+    return str(conf), str(inp / "ElemDepart.dat")
+
+
+def test_reader_reproduces_reference_grid_setup(tmp_path):
+    d = np.load(os.path.join(GOLD, "naca_grid.npz"))
+    conf, part = write_reference_mesh(tmp_path, d)
+    g = Grid.read(conf, part, box=15.0)
+    E = g.n_elements
+    assert E == d["tri"].shape[0]
+    assert np.array_equal(g.tri, d["tri"]) and np.array_equal(g.neighbour, d["neigh"])
+    assert np.array_equal(g.partition, d["partition"])
+    rn, rse, rflag = d["ref_neighbour"], d["ref_shared_edge"], d["ref_flag"]
+    for e in range(E):
+        for k in range(3):
+            if g.neighbour[e, k] >= 0:
+                assert g.shared_edge[e, k] == rse[k, e]
+            else:
+                ghost = rn[k, e]
+                assert g.boundary[e, k] == rflag[ghost]
+    assert (g.boundary[g.neighbour < 0] == L.BC_FARFIELD).sum() == 64
+    assert (g.boundary[g.neighbour < 0] == L.BC_REFLECTION).sum() == 130
+
+
+def test_reader_rejects_inconsistent_edges(tmp_path):
+    d = dict(np.load(os.path.join(GOLD, "naca_grid.npz")))
+    nb = d["neigh"].copy()
+    e = int(np.nonzero((nb >= 0).all(axis=1))[0][0])
+    nb[e, 0], nb[e, 1] = nb[e, 1], nb[e, 0]
+    d["neigh"] = nb
+    conf, part = write_reference_mesh(tmp_path, d)
+    with pytest.raises(L.RkdgError) as ei:
+        Grid.read(conf, part)
+    assert ei.value.code == L.ERR_MESH
+
+
+def test_reader_missing_file(tmp_path):
+    with pytest.raises(L.RkdgError) as ei:
+        Grid.read(str(tmp_path / "nope.conf"))
+    assert ei.value.code == L.ERR_IO
+
+
+def config_get(path, key):
+    lib = L.load()
+    buf = ctypes.create_string_buffer(4096)
+    L.check(lib.rkdg_config_get(path.encode(), key.encode(), buf, 4096), None, "config")
+    return buf.value.decode("latin-1")
+
+
+def test_config_parser_matches_reference_values(tmp_path):
+    with open(os.path.join(GOLD, "config_values.json")) as f:
+        gold = json.load(f)
+    for name, entry in gold.items():
+        p = tmp_path / (name + ".txt")
+        p.write_text(entry["text"])
+        for k, v in zip(entry["keys"], entry["values"]):
+            assert config_get(str(p), k) == v, (name, k)
+
+
+def cxx_g(v):
+    """C++ ostream default formatting (%g, 6 significant digits)."""
+    return "%g" % v
+
+
+def test_plt_writer_format(tmp_path):
+    g = Grid.splitquad(3, 2, 0.0, 0.0, 1.5, 1.0, periodic=False, diagonal=0)
+    E, NV = g.n_elements, g.n_vertices
+    rng = np.random.default_rng(0)
+    U = np.zeros((4, 3, E))
+    U[0, 0] = rng.uniform(0.9, 1.1, E)
+    U[1, 0] = rng.uniform(-0.1, 0.1, E)
+    U[2, 0] = rng.uniform(-0.1, 0.1, E)
+    U[3, 0] = rng.uniform(2.4, 2.6, E)
+    path = str(tmp_path / "s.plt")
+    g.write_plt(path, U, 3)
+    txt = open(path).read()
+
+    def block(vals):
+        s = ""
+        for i, v in enumerate(vals):
+            s += v + "  "
+            if i % 6 == 0:
+                s += "\n"
+        return s
+
+    r, u, v = U[0, 0], U[1, 0] / U[0, 0], U[2, 0] / U[0, 0]
+    p = 0.4 * (U[3, 0] - 0.5 * r * (u * u + v * v))
+    ma = np.sqrt(u * u + v * v) / np.sqrt(1.4 * p / r)
+    exp = ("TITLE=RKDG\nVARIABLES=X , Y , rho , u , v , p, Ma , FLAG\n"
+           f"ZONE T= T1 N= {NV} , E= {E} , ZONETYPE=FETRIANGLE\nDATAPACKING=BLOCK\n"
+           "VARLOCATION=([1-2]=NODAL,[3-8]=CELLCENTERED)\nDT=(SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE)\n")
+    exp += block([cxx_g(x) for x in g.vx]) + block([cxx_g(y) for y in g.vy]) + "\n"
+    for arr in (r, u, v, p, ma):
+        exp += block([cxx_g(x) for x in arr]) + "\n"
+    exp += block(["1"] * E) + "\n"
+    for t in g.tri:
+        exp += f"{t[0] + 1}    {t[1] + 1}    {t[2] + 1}\n"
+    assert txt == exp
+
+
+def test_state_dump_round_trip(tmp_path):
+    lib = L.load()
+    U = np.random.default_rng(1).normal(size=(4, 6, 33))
+    path = str(tmp_path / "st.bin").encode()
+    L.check(lib.rkdg_dump_state(path, 6, 33, 1.25, L.dptr(U)), None, "dump")
+    V = np.zeros_like(U)
+    t = ctypes.c_double()
+    L.check(lib.rkdg_load_state(path, 6, 33, ctypes.byref(t), L.dptr(V)), None, "load")
+    assert np.array_equal(U, V) and t.value == 1.25
+    assert lib.rkdg_load_state(path, 6, 34, ctypes.byref(t), L.dptr(V)) == L.ERR_IO
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_basis_exchange_round_trip_and_mean(P):
+    """element -> native -> element recovers the reference coefficients; the mode-0
+    coefficient (cell mean) is the same in both bases."""
+    lib = L.load()
+    g = Grid.splitquad(6, 5, 0.0, 0.0, 3.0, 2.0, periodic=False, diagonal=1, jitter=0.2, seed=1)
+    ex, ey = g.element_coords()
+    ex, ey = np.ascontiguousarray(ex), np.ascontiguousarray(ey)
+    Np = (P + 1) * (P + 2) // 2
+    E = g.n_elements
+    U = np.random.default_rng(P).normal(size=(4, Np, E))
+    D = np.zeros_like(U)
+    B = np.zeros_like(U)
+    L.check(lib.rkdg_convert_basis(P, E, L.dptr(ex), L.dptr(ey), L.BASIS_ELEMENT, L.dptr(U), L.dptr(D)), None, "c")
+    L.check(lib.rkdg_convert_basis(P, E, L.dptr(ex), L.dptr(ey), L.BASIS_NATIVE, L.dptr(D), L.dptr(B)), None, "c")
+    assert np.allclose(B, U, rtol=1e-12, atol=1e-12 * np.abs(U).max())
+    assert np.allclose(D[:, 0], U[:, 0], rtol=1e-13, atol=1e-14)
+
+
+def test_freestream_state_matches_reference_formula():
+    fs = freestream_state(1.4, 5.0, 0.4, 1.0, 1.0)
+    ut = np.sqrt(1.4) * 0.4
+    al = 5.0 * np.arctan(1.0) * 4 / 180
+    assert np.allclose(fs, [1.0, ut * np.cos(al), ut * np.sin(al), 0.5 * ut * ut + 1 / 0.4], rtol=1e-15)
