@@ -58,7 +58,7 @@ extern "C" {
 
 #define RKDG_IC_FREESTREAM 0  /* src/cudarkdgsolver.cu:326-368; p: gamma, alpha_rad, mach, rhoref, pref */
 #define RKDG_IC_VORTEX     1  /* isentropic vortex; p: gamma, beta, uinf, vinf, xc, yc, Lx, Ly, t (Lx=Ly=0: no images) */
-#define RKDG_IC_RIEMANN4   2  /* four quadrants; p: gamma, xs, ys, then (rho,u,v,p) for UR, UL, LL, LR */
+#define RKDG_IC_RIEMANN4   2  /* four quadrants (cell averages); p: gamma, xs, ys, then (rho,u,v,p) for UR, UL, LL, LR */
 
 /* ---- mesh and parameters ----------------------------------------------------------- */
 typedef struct rkdg_mesh {

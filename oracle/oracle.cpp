@@ -24,7 +24,8 @@
 //   * P1 = first three modes of the P2 construction; P3 = the same Gram-Schmidt continued
 //     to cubic monomials, 12-point Dunavant volume rule, 4-point Gauss edge rule;
 //   * Roe flux (Roe 1981) with the Harten-Hyman entropy fix on the acoustic waves;
-//   * TVB minmod limiter (Cockburn & Shu 1998, triangles), component-wise;
+//   * TVB minmod limiter (Cockburn & Shu 1998, triangles), component-wise, followed by the
+//     positivity-preserving scaling of Zhang & Shu (2010);
 //   * boundary handling per edge (the reference creates one ghost per element);
 //   * L2 projection of the isentropic vortex and of the four-quadrant Riemann data.
 //
@@ -723,6 +724,56 @@ void residual(Solver& S, const std::vector<double>& A, std::vector<double>& Rout
     }
 }
 
+// Positivity-preserving scaling (Zhang & Shu 2010), applied after the TVB limiter:
+// the modes >= 1 are scaled toward the cell mean so that density, then pressure, stay
+// >= eps at every volume and edge quadrature point.  Pressure is concave in the
+// conserved variables, so theta = (p_mean - eps) / (p_mean - p_q) is a valid bound.
+constexpr double POS_EPS = 1e-13;
+
+void positivity(Solver& S, std::vector<double>& A) {
+    const int E = S.E, Np = S.Np, Nqv = S.Nqv, NE = 3 * S.Nqe;
+    const double g = S.prm.gamma;
+    #pragma omp parallel for schedule(static)
+    for (int e = 0; e < E; ++e) {
+        const ElemInfo& I = S.info[e];
+        double c[4][MAXNP];
+        for (int v = 0; v < 4; ++v)
+            for (int i = 0; i < Np; ++i) c[v][i] = A[((size_t)v * Np + i) * E + e];
+        auto point = [&](int q, double* U) {
+            const double* bf;
+            for (int v = 0; v < 4; ++v) {
+                double s = c[v][0];
+                for (int i = 1; i < Np; ++i) {
+                    bf = q < Nqv ? &I.vol_bf[i][q] : &I.edge_bf[i][q - Nqv];
+                    s += *bf * c[v][i];
+                }
+                U[v] = s;
+            }
+        };
+        const double rm = c[0][0];
+        double rmin = rm;
+        for (int q = 0; q < Nqv + NE; ++q) { double U[4]; point(q, U); rmin = std::min(rmin, U[0]); }
+        double th1 = 1.0;
+        if (rmin < POS_EPS) th1 = std::min(1.0, (rm - POS_EPS) / (rm - rmin));
+        if (th1 < 1.0)
+            for (int i = 1; i < Np; ++i) c[0][i] *= th1;
+        const double pm = (g - 1) * (c[3][0] - 0.5 * (c[1][0] * c[1][0] + c[2][0] * c[2][0]) / rm);
+        double th2 = 1.0;
+        for (int q = 0; q < Nqv + NE; ++q) {
+            double U[4];
+            point(q, U);
+            double pq = (g - 1) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0]);
+            if (pq < POS_EPS) th2 = std::min(th2, (pm - POS_EPS) / (pm - pq));
+        }
+        if (th1 < 1.0 || th2 < 1.0) {
+            for (int v = 0; v < 4; ++v)
+                for (int i = 1; i < Np; ++i)
+                    A[((size_t)v * Np + i) * E + e] = (v == 0 ? c[0][i] : c[v][i]) * (th2 < 1.0 ? th2 : 1.0);
+            S.flags[e] |= 2;
+        }
+    }
+}
+
 // TVB minmod limiter on the P1 part (Cockburn & Shu 1998), component-wise, in place.
 void limit(Solver& S, std::vector<double>& A) {
     const int E = S.E, Np = S.Np;
@@ -780,6 +831,7 @@ void limit(Solver& S, std::vector<double>& A) {
         }
         S.flags[e] = troubled ? 1 : 0;
     }
+    positivity(S, A);
 }
 
 double compute_dt(Solver& S, const std::vector<double>& A) {
@@ -992,7 +1044,8 @@ int orc_residual_change(void* h, double* res) {
 // L2 projection of initial data onto each element's basis (36-point collapsed rule).
 // kind 0: free stream of src/cudarkdgsolver.cu:326-368 (prm: gamma, alpha_rad, mach, rhoref, pref)
 // kind 1: isentropic vortex (prm: gamma, beta, uinf, vinf, xc, yc, Lx, Ly, t)
-// kind 2: four-quadrant Riemann data (prm: gamma, xs, ys, 4 x (rho,u,v,p) for UR, UL, LL, LR)
+// kind 2: four-quadrant Riemann data (prm: gamma, xs, ys, 4 x (rho,u,v,p) for UR, UL, LL, LR),
+//         cell averages only
 int orc_project(void* h, const double* ex, const double* ey, int kind, const double* prm) {
     Solver* S = (Solver*)h;
     const int E = S->E, Np = S->Np;
@@ -1036,8 +1089,10 @@ int orc_project(void* h, const double* ex, const double* ey, int kind, const dou
                     den[i] += w * bf * bf;
                 }
             }
+        // discontinuous four-quadrant data start from cell averages (no slopes)
         for (int i = 0; i < Np; ++i)
-            for (int v = 0; v < 4; ++v) S->U[((size_t)v * Np + i) * E + e] = num[i][v] / den[i];
+            for (int v = 0; v < 4; ++v)
+                S->U[((size_t)v * Np + i) * E + e] = (kind == 2 && i > 0) ? 0.0 : num[i][v] / den[i];
     }
     return 0;
 }

@@ -401,8 +401,10 @@ void project_native(int P, const double* x, const double* y, int kind, const dou
             den[j] += R.w[q] * s * s;
         }
     }
+    // discontinuous four-quadrant data start from cell averages: the L2 projection of a
+    // jump onto P_k over- and undershoots and can make the pressure negative at flux points
     for (int v = 0; v < NVAR; ++v)
-        for (int j = 0; j < Np; ++j) d[v * Np + j] = num[v][j] / den[j];
+        for (int j = 0; j < Np; ++j) d[v * Np + j] = (kind == RKDG_IC_RIEMANN4 && j > 0) ? 0.0 : num[v][j] / den[j];
 }
 
 // Cockburn-Shu choice of two neighbour barycentres per edge midpoint: m_k - b0 =

@@ -120,7 +120,7 @@ static int launch_one_stage(rkdg_ctx* c, int s, int slot, int dt_mode, double dt
     if (c->prm.limiter == RKDG_LIMITER_TVB) {
         LimArgs la{};
         la.U = a.Uout; la.lim = c->lim; la.pairs = c->pairs; la.nbr = c->nbr; la.flags = c->flags;
-        la.ld = c->ld; la.E = c->E; la.M = c->prm.tvb_M;
+        la.ld = c->ld; la.E = c->E; la.M = c->prm.tvb_M; la.gamma = c->prm.gamma;
         rc = launch_limiter(c->P, la, c->block, c->stream);
         if (rc) return fail(c, RKDG_ERR_HIP, std::string("limiter kernel: ") + hip_error_string(rc));
     }
@@ -672,7 +672,7 @@ int rkdg_time_advance(rkdg_ctx* c, int32_t nsteps, double* out) {
             if (lim) {
                 LimArgs la{};
                 la.U = a.Uout; la.lim = c->lim; la.pairs = c->pairs; la.nbr = c->nbr; la.flags = c->flags;
-                la.ld = c->ld; la.E = c->E; la.M = c->prm.tvb_M;
+                la.ld = c->ld; la.E = c->E; la.M = c->prm.tvb_M; la.gamma = c->prm.gamma;
                 HIPCHK(c, hipEventRecord(ev[2 + 2 * n + 2 * k], c->stream));
                 rc = launch_limiter(c->P, la, c->block, c->stream);
                 if (rc) return fail(c, RKDG_ERR_HIP, hip_error_string(rc));

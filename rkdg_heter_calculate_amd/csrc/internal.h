@@ -139,6 +139,7 @@ struct LimArgs {
     long long ld;
     int E;
     double M;
+    double gamma;
 };
 
 int upload_tables(int P, const DevTables& t);  // per current device

@@ -20,6 +20,7 @@
 // XCD sweeps one contiguous range of elements and neighbour gathers hit its own L2.
 #include <hip/hip_runtime.h>
 #include <climits>
+#include <cstdlib>
 #include "internal.h"
 
 namespace rkdg {
@@ -67,6 +68,39 @@ __device__ __forceinline__ void euler_fg(double g, const double* U, double* F, d
     const double p = (g - 1.0) * (U[3] - 0.5 * U[0] * (u * u + v * v));
     F[0] = U[1]; F[1] = U[1] * u + p; F[2] = U[1] * v; F[3] = u * (U[3] + p);
     G[0] = U[2]; G[1] = U[2] * u; G[2] = U[2] * v + p; G[3] = v * (U[3] + p);
+}
+
+// 1/x from the hardware reciprocal refined by two Newton steps (about 1 ulp), in place
+// of the IEEE division sequence.
+__device__ __forceinline__ double frcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double t = fma(-x, r, 1.0);
+    r = fma(t, r, r);
+    t = fma(-x, r, 1.0);
+    return fma(t, r, r);
+}
+
+__device__ __forceinline__ void euler_fg_fast(double g, const double* U, double* F, double* G) {
+    const double ir = frcp(U[0]);
+    const double u = U[1] * ir, v = U[2] * ir;
+    const double p = (g - 1.0) * (U[3] - 0.5 * (U[1] * u + U[2] * v));
+    F[0] = U[1]; F[1] = U[1] * u + p; F[2] = U[1] * v; F[3] = u * (U[3] + p);
+    G[0] = U[2]; G[1] = U[2] * u; G[2] = U[2] * v + p; G[3] = v * (U[3] + p);
+}
+
+// Flux of the Euler equations in direction (a, b): F a + G b, for any (a, b) (a unit
+// normal on edges, a row of the inverse Jacobian in the volume).
+__device__ __forceinline__ void euler_dir(double g, const double* U, double ir, double pr, double a, double b,
+                                          double* Fd) {
+    const double un = (U[1] * a + U[2] * b) * ir;
+    Fd[0] = U[1] * a + U[2] * b;
+    Fd[1] = fma(U[1], un, pr * a);
+    Fd[2] = fma(U[2], un, pr * b);
+    Fd[3] = un * (U[3] + pr);
+}
+
+__device__ __forceinline__ double pressure(double g, const double* U, double ir) {
+    return (g - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) * ir);
 }
 
 // Roe (1981) normal flux with the Harten-Hyman (1983) entropy fix on the acoustic waves.
@@ -191,12 +225,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
     double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
 
     if (e < a.E) {
-        // geometry: inverse Jacobian, area, edge lengths, outward normals
-        double geo[GEO_ROWS];
-#pragma unroll
-        for (int r = 0; r < GEO_ROWS; ++r) geo[r] = a.geo[r * ld + e];
-        const double xix = geo[0], xiy = geo[1], etx = geo[2], ety = geo[3], area = geo[4];
-
         double Uo[4][Np], R[4][Np];
 #pragma unroll
         for (int v = 0; v < 4; ++v)
@@ -205,52 +233,49 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
                 Uo[v][i] = a.U[(v * Np + i) * ld + e];
                 R[v][i] = 0.0;
             }
+        const double area = a.geo[4 * ld + e];
 
         // ---- volume integral: sum_q w_q A (F . grad psi_i) -------------------------
+        // The point loop stays rolled: each iteration reads one row of the reference
+        // operators into scalar registers instead of keeping every table live.
+        {
+            const double xix = a.geo[0 * ld + e], xiy = a.geo[1 * ld + e];
+            const double etx = a.geo[2 * ld + e], ety = a.geo[3 * ld + e];
+#pragma unroll 1
+            for (int q = 0; q < Nqv; ++q) {
+                double uq[4];
 #pragma unroll
-        for (int q = 0; q < Nqv; ++q) {
-            double uq[4];
+                for (int v = 0; v < 4; ++v) {
+                    double s = Uo[v][0] * T.Vv[q][0];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                double s = 0.0;
+                    for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Vv[q][i], s);
+                    uq[v] = s;
+                }
+                // contravariant fluxes w A F.(xi_x, xi_y) and w A F.(eta_x, eta_y)
+                const double ir = frcp(uq[0]);
+                const double pr = pressure(g, uq, ir);
+                const double wa = T.wv[q] * area;
+                double Fx[4], Fe[4];
+                euler_dir(g, uq, ir, pr, wa * xix, wa * xiy, Fx);
+                euler_dir(g, uq, ir, pr, wa * etx, wa * ety, Fe);
 #pragma unroll
-                for (int i = 0; i < Np; ++i) s += Uo[v][i] * T.Vv[q][i];
-                uq[v] = s;
-            }
-            double F[4], G[4];
-            euler_fg(g, uq, F, G);
-            const double wa = T.wv[q] * area;
+                for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const double fc = wa * (F[v] * xix + G[v] * xiy);
-                const double gc = wa * (F[v] * etx + G[v] * ety);
-#pragma unroll
-                for (int i = 1; i < Np; ++i) R[v][i] += fc * T.Dxi[q][i] + gc * T.Deta[q][i];
+                    for (int i = 1; i < Np; ++i) R[v][i] = fma(Fx[v], T.Dxi[q][i], fma(Fe[v], T.Deta[q][i], R[v][i]));
             }
         }
 
         // ---- surface integral, element-owned -------------------------------------
-#pragma unroll
+#pragma unroll 1
         for (int k = 0; k < 3; ++k) {
             const int w = a.nbr[k * ld + e];
-            const double lk = geo[5 + k], nx = geo[8 + 2 * k], ny = geo[9 + 2 * k];
-            double uL[Nqe][4], uR[Nqe][4], mR[4];
-#pragma unroll
-            for (int j = 0; j < Nqe; ++j)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int i = 0; i < Np; ++i) s += Uo[v][i] * T.Ve[k][j][i];
-                    uL[j][v] = s;
-                }
+            const double lk = a.geo[(5 + k) * ld + e];
+            const double nx = a.geo[(8 + 2 * k) * ld + e], ny = a.geo[(9 + 2 * k) * ld + e];
+            double uR[Nqe][4], mR[4];
             if (w >= 0) {
+                // neighbour trace at the mirrored points, streamed over its modes
                 const int n = w >> 2, sn = w & 3;
                 const double* tb = sVe + sn * (Nqe * Np);
-#pragma unroll
-                for (int j = 0; j < Nqe; ++j)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) uR[j][v] = 0.0;
 #pragma unroll
                 for (int i = 0; i < Np; ++i) {
                     double c[4];
@@ -264,7 +289,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
                     for (int j = 0; j < Nqe; ++j) {
                         const double t = tb[(Nqe - 1 - j) * Np + i];
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) uR[j][v] += c[v] * t;
+                        for (int v = 0; v < 4; ++v) uR[j][v] = i == 0 ? c[v] * t : fma(c[v], t, uR[j][v]);
                     }
                 }
             } else if (w == -RKDG_BC_FARFIELD) {
@@ -275,65 +300,79 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
 #pragma unroll
                     for (int j = 0; j < Nqe; ++j) uR[j][v] = a.fs[v];
                 }
-            } else if (w == -RKDG_BC_REFLECTION) {
-                // CELL_REFLECTION (232-268): velocity mirrored about the edge normal
-                const double cxx = ny * ny - nx * nx, cyy = nx * nx - ny * ny, cxy = 2.0 * nx * ny;
+            } else {
+                // CELL_REFLECTION (232-268) mirrors the velocity about the edge normal;
+                // CELL_OUTFLOW (203-230) copies the interior trace
+                const bool refl = w == -RKDG_BC_REFLECTION;
+                const double cxx = refl ? ny * ny - nx * nx : 1.0, cyy = refl ? nx * nx - ny * ny : 1.0;
+                const double cxy = refl ? 2.0 * nx * ny : 0.0;
 #pragma unroll
                 for (int j = 0; j < Nqe; ++j) {
-                    uR[j][0] = uL[j][0];
-                    uR[j][3] = uL[j][3];
-                    uR[j][1] = uL[j][1] * cxx - cxy * uL[j][2];
-                    uR[j][2] = uL[j][2] * cyy - cxy * uL[j][1];
+                    double uL[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        double s = Uo[v][0] * T.Ve[k][j][0];
+#pragma unroll
+                        for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Ve[k][j][i], s);
+                        uL[v] = s;
+                    }
+                    uR[j][0] = uL[0];
+                    uR[j][3] = uL[3];
+                    uR[j][1] = uL[1] * cxx - cxy * uL[2];
+                    uR[j][2] = uL[2] * cyy - cxy * uL[1];
                 }
                 mR[0] = Uo[0][0];
                 mR[3] = Uo[3][0];
                 mR[1] = Uo[1][0] * cxx - cxy * Uo[2][0];
                 mR[2] = Uo[2][0] * cyy - cxy * Uo[1][0];
-            } else {
-                // CELL_OUTFLOW (203-230): transmissive
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    mR[v] = Uo[v][0];
-#pragma unroll
-                    for (int j = 0; j < Nqe; ++j) uR[j][v] = uL[j][v];
-                }
             }
 
             double alpha = 0.0;
             if (FLUX == RKDG_FLUX_LF) {
                 // kernel_calculateLFCoeff (src/kernelfunctions.cu:370-387)
                 const double rho = 0.5 * (Uo[0][0] + mR[0]);
-                const double u = 0.5 * (Uo[1][0] + mR[1]) / rho;
-                const double v = 0.5 * (Uo[2][0] + mR[2]) / rho;
+                const double ir = frcp(rho);
+                const double u = 0.5 * (Uo[1][0] + mR[1]) * ir;
+                const double v = 0.5 * (Uo[2][0] + mR[2]) * ir;
                 const double rE = 0.5 * (Uo[3][0] + mR[3]);
                 const double p = (g - 1.0) * (rE - 0.5 * rho * (u * u + v * v));
-                alpha = fabs(u * nx + v * ny) + sqrt(g * p / rho);
+                alpha = fabs(u * nx + v * ny) + sqrt(g * p * ir);
             }
+            const double hl = 0.5 * lk;
 #pragma unroll
             for (int j = 0; j < Nqe; ++j) {
+                double uL[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double s = Uo[v][0] * T.Ve[k][j][0];
+#pragma unroll
+                    for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Ve[k][j][i], s);
+                    uL[v] = s;
+                }
                 double Fh[4];
                 if (FLUX == RKDG_FLUX_LF) {
-                    double FL[4], GL[4], FR[4], GR[4];
-                    euler_fg(g, uL[j], FL, GL);
-                    euler_fg(g, uR[j], FR, GR);
+                    // 0.5 (F(uL).n + F(uR).n - alpha (uR - uL)), kernel_calculateFlux's formula
+                    double FL[4], FR[4];
+                    const double irl = frcp(uL[0]), irr = frcp(uR[j][0]);
+                    euler_dir(g, uL, irl, pressure(g, uL, irl), nx, ny, FL);
+                    euler_dir(g, uR[j], irr, pressure(g, uR[j], irr), nx, ny, FR);
 #pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        Fh[v] = 0.5 * ((FL[v] + FR[v]) * nx + (GL[v] + GR[v]) * ny - alpha * (uR[j][v] - uL[j][v]));
+                    for (int v = 0; v < 4; ++v) Fh[v] = 0.5 * (FL[v] + FR[v] - alpha * (uR[j][v] - uL[v]));
                 } else {
-                    roe_flux(g, uL[j], uR[j], nx, ny, Fh);
+                    roe_flux(g, uL, uR[j], nx, ny, Fh);
                 }
-                const double wj = T.we[j] * 0.5 * lk;
+                const double wj = T.we[j] * hl;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const double f = wj * Fh[v];
 #pragma unroll
-                    for (int i = 0; i < Np; ++i) R[v][i] -= f * T.Ve[k][j][i];
+                    for (int i = 0; i < Np; ++i) R[v][i] = fma(-f, T.Ve[k][j][i], R[v][i]);
                 }
             }
         }
 
         // ---- SSP-RK3 stage (src/kernelfunctions.cu:640-748) -----------------------
-        const double ia = 1.0 / area;
+        const double ia = frcp(area);
         double mean[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v)
@@ -347,15 +386,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
                 } else if (a.stage == 1) {
                     out = 0.75 * a.U0[idx] + 0.25 * Uo[v][i] + 0.25 * dt * r;
                 } else {
-                    if (i == 0) {
-                        const double o0 = a.U0[idx];
-                        out = (1.0 / 3.0) * o0 + (2.0 / 3.0) * Uo[v][i] + (2.0 / 3.0) * dt * r;
-                        if (a.dtflag[e]) {
-                            if (v == 0) r0_l = fabs(out - o0);
-                            if (v == 3) r1_l = fabs(out - o0);
-                        }
-                    } else {
-                        out = (1.0 / 3.0) * a.U0[idx] + (2.0 / 3.0) * Uo[v][i] + (2.0 / 3.0) * dt * r;
+                    const double o0 = a.U0[idx];
+                    out = (1.0 / 3.0) * o0 + (2.0 / 3.0) * Uo[v][i] + (2.0 / 3.0) * dt * r;
+                    if (i == 0 && a.dtflag[e]) {
+                        if (v == 0) r0_l = fabs(out - o0);
+                        if (v == 3) r1_l = fabs(out - o0);
                     }
                 }
                 if (i == 0) mean[v] = out;
@@ -369,7 +404,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
             const double p = (g - 1.0) * (mean[3] - 0.5 * rho * (u * u + v * v));
             if (!(rho > 0.0) || !(p > 0.0)) atomicMin(&ti.st->err_pos, e);
             const double c = sqrt(g * p / rho);
-            const double s = (sqrt(u * u + v * v) + c) * (geo[5] + geo[6] + geo[7]) / area;
+            const double per = a.geo[5 * ld + e] + a.geo[6 * ld + e] + a.geo[7 * ld + e];
+            const double s = (sqrt(u * u + v * v) + c) * per / area;
             if (a.dtflag[e] && s > 0.0) ws_l = s;
         }
     }
@@ -377,8 +413,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
         block_max3_atomic(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], &ti.st->res[0], &ti.st->res[1], true);
 }
 
-template <int P, int FLUX>
-__global__ void __launch_bounds__(256) k_stage(StageArgs a) {
+// WPE > 0 asks the compiler for at least WPE waves per SIMD (fewer registers, possibly
+// spills); chosen at run time by measurement.
+template <int P, int FLUX, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8))) k_stage(StageArgs a) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqe = D::Nqe;
     const DevTables& T = c_tab[P - 1];
@@ -436,23 +474,25 @@ __global__ void k_state_init(DevState* st, int n, int slot, double t) {
     }
 }
 
-// TVB minmod limiter (Cockburn & Shu 1998) on the P1 part, component-wise, in place.
-// Reads the neighbours' means only; the means are never modified, so the in-place
-// update is race-free.
+// TVB minmod limiter (Cockburn & Shu 1998) on the P1 part, component-wise, followed by
+// the positivity-preserving scaling of Zhang & Shu (2010), in place.  The limiter reads
+// the neighbours' means only and never modifies a mean, so the in-place update is
+// race-free.  Flags: bit 0 TVB-limited, bit 1 positivity-scaled.
+constexpr double POS_EPS = 1e-13;
+
 template <int P>
 __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
-    constexpr int Np = Deg<P>::Np;
+    using D = Deg<P>;
+    constexpr int Np = D::Np, Nqv = D::Nqv, Nqe = D::Nqe;
     const DevTables& T = c_tab[P - 1];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
     const long long ld = a.ld;
-    double m0[4], s1[4], s2[4], nm[3][4];
+    double c[4][Np], nm[3][4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        m0[v] = a.U[(v * Np + 0) * ld + e];
-        s1[v] = a.U[(v * Np + 1) * ld + e];
-        s2[v] = a.U[(v * Np + 2) * ld + e];
-    }
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int i = 0; i < Np; ++i) c[v][i] = a.U[(v * Np + i) * ld + e];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int w = a.nbr[k * ld + e];
@@ -462,7 +502,7 @@ __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
             for (int v = 0; v < 4; ++v) nm[k][v] = a.U[(v * Np) * ld + n];
         } else {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) nm[k][v] = m0[v];
+            for (int v = 0; v < 4; ++v) nm[k][v] = c[v][0];
         }
     }
     double al[6];
@@ -471,19 +511,20 @@ __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
     const double tol = a.M * a.lim[6 * ld + e];
     const int pk = a.pairs[e];
     const double nu = 1.5;
-    int troubled = 0;
+    int flag = 0;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
+        const double m0 = c[v][0];
         double d[3];
         bool changed = false;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double ut = T.Vmid[k][1] * s1[v] + T.Vmid[k][2] * s2[v];
+            const double ut = T.Vmid[k][1] * c[v][1] + T.Vmid[k][2] * c[v][2];
             const int k1 = (pk >> (4 * k)) & 3, k2 = (pk >> (4 * k + 2)) & 3;
             double n1 = nm[0][v], n2 = nm[0][v];
             if (k1 == 1) n1 = nm[1][v]; else if (k1 == 2) n1 = nm[2][v];
             if (k2 == 1) n2 = nm[1][v]; else if (k2 == 2) n2 = nm[2][v];
-            const double b = nu * (al[2 * k] * (n1 - m0[v]) + al[2 * k + 1] * (n2 - m0[v]));
+            const double b = nu * (al[2 * k] * (n1 - m0) + al[2 * k + 1] * (n2 - m0));
             double r;
             if (fabs(ut) <= tol) r = ut;
             else if (ut > 0 && b > 0) r = fmin(ut, b);
@@ -493,7 +534,7 @@ __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
             d[k] = r;
         }
         if (!changed) continue;
-        troubled = 1;
+        flag = 1;
         const double sum = d[0] + d[1] + d[2];
         if (sum != 0.0) {
             double pos = 0, neg = 0;
@@ -504,12 +545,57 @@ __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) d[k] = tp * fmax(0.0, d[k]) - tn * fmax(0.0, -d[k]);
         }
-        a.U[(v * Np + 1) * ld + e] = T.recon[0][0] * d[0] + T.recon[0][1] * d[1] + T.recon[0][2] * d[2];
-        a.U[(v * Np + 2) * ld + e] = T.recon[1][0] * d[0] + T.recon[1][1] * d[1] + T.recon[1][2] * d[2];
+        c[v][1] = T.recon[0][0] * d[0] + T.recon[0][1] * d[1] + T.recon[0][2] * d[2];
+        c[v][2] = T.recon[1][0] * d[0] + T.recon[1][1] * d[1] + T.recon[1][2] * d[2];
 #pragma unroll
-        for (int i = 3; i < Np; ++i) a.U[(v * Np + i) * ld + e] = 0.0;
+        for (int i = 3; i < Np; ++i) c[v][i] = 0.0;
     }
-    a.flags[e] = troubled;
+    // positivity: density first, then pressure, at every volume and edge point
+    const double g = a.gamma;
+    const double rm = c[0][0];
+    double rmin = rm;
+#pragma unroll
+    for (int q = 0; q < Nqv + 3 * Nqe; ++q) {
+        double s = c[0][0];
+#pragma unroll
+        for (int i = 1; i < Np; ++i) s += (q < Nqv ? T.Vv[q][i] : T.Ve[(q - Nqv) / Nqe][(q - Nqv) % Nqe][i]) * c[0][i];
+        rmin = fmin(rmin, s);
+    }
+    double th1 = 1.0;
+    if (rmin < POS_EPS) th1 = fmin(1.0, (rm - POS_EPS) / (rm - rmin));
+    if (th1 < 1.0) {
+#pragma unroll
+        for (int i = 1; i < Np; ++i) c[0][i] *= th1;
+    }
+    const double pm = (g - 1.0) * (c[3][0] - 0.5 * (c[1][0] * c[1][0] + c[2][0] * c[2][0]) / rm);
+    double th2 = 1.0;
+#pragma unroll
+    for (int q = 0; q < Nqv + 3 * Nqe; ++q) {
+        double U[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double s = c[v][0];
+#pragma unroll
+            for (int i = 1; i < Np; ++i) s += (q < Nqv ? T.Vv[q][i] : T.Ve[(q - Nqv) / Nqe][(q - Nqv) % Nqe][i]) * c[v][i];
+            U[v] = s;
+        }
+        const double pq = (g - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0]);
+        if (pq < POS_EPS) th2 = fmin(th2, (pm - POS_EPS) / (pm - pq));
+    }
+    if (th1 < 1.0 || th2 < 1.0) {
+        flag |= 2;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 1; i < Np; ++i) c[v][i] *= th2;
+    }
+    if (flag) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 1; i < Np; ++i) a.U[(v * Np + i) * ld + e] = c[v][i];
+    }
+    a.flags[e] = flag;
 }
 
 int upload_tables(int P, const DevTables& t) {
@@ -524,11 +610,28 @@ static int grid_blocks(const StageArgs& a) {
     return nb;
 }
 
+static int stage_wpe() {
+    static int w = [] {
+        const char* e = std::getenv("RKDG_WPE");
+        return e ? std::atoi(e) : 0;
+    }();
+    return w;
+}
+
+template <int P, int FLUX>
+static void launch_stage_pf(int block, const StageArgs& a, hipStream_t s) {
+    dim3 grid(grid_blocks(a)), blk(block);
+    switch (stage_wpe()) {
+        case 2: hipLaunchKernelGGL((k_stage<P, FLUX, 2>), grid, blk, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_stage<P, FLUX, 3>), grid, blk, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_stage<P, FLUX, 0>), grid, blk, 0, s, a); break;
+    }
+}
+
 template <int P>
 static void launch_stage_p(int flux, int block, const StageArgs& a, hipStream_t s) {
-    dim3 grid(grid_blocks(a)), blk(block);
-    if (flux == RKDG_FLUX_ROE) hipLaunchKernelGGL((k_stage<P, RKDG_FLUX_ROE>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((k_stage<P, RKDG_FLUX_LF>), grid, blk, 0, s, a);
+    if (flux == RKDG_FLUX_ROE) launch_stage_pf<P, RKDG_FLUX_ROE>(block, a, s);
+    else launch_stage_pf<P, RKDG_FLUX_LF>(block, a, s);
 }
 
 int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream) {

@@ -305,3 +305,38 @@ def test_tvb_limiter_identities():
     o2.set_state(Ur)
     o2.stage(0, 0.0)
     assert np.array_equal(o2.get_state(), Ur) and o2.flags().sum() == 0
+
+
+def test_positivity_scaling_keeps_point_pressure_positive():
+    """After the limiter pass, density and pressure at every volume and edge point are
+    positive, also for data whose unlimited traces are not (P1, low-pressure state)."""
+    g = Grid.splitquad(10, 10, 0.0, 0.0, 1.0, 1.0, periodic=False, diagonal=1, jitter=0.2, seed=3, bc=BC_OUTFLOW)
+    E = g.n_elements
+    ex, ey = g.element_coords()
+    rng = np.random.default_rng(5)
+    U = np.zeros((4, 3, E))
+    r = rng.uniform(0.1, 0.2, E)
+    u = rng.uniform(1.0, 1.3, E)
+    v = rng.uniform(1.0, 1.3, E)
+    p = rng.uniform(0.02, 0.04, E)
+    U[0, 0], U[1, 0], U[2, 0], U[3, 0] = r, r * u, r * v, p / 0.4 + 0.5 * r * (u * u + v * v)
+    U[:, 1:, :] = rng.normal(0, 1, (4, 2, E)) * np.abs(U[:, :1, :]) * 0.8
+    o = oracle_for(g, P=1, limiter=1, tvb_M=1e30, dt_policy=2, dt_fixed=1.0)  # TVB inactive
+    o.set_state(U)
+    o.stage(0, 0.0)
+    V = o.get_state()
+    assert np.array_equal(V[:, 0], U[:, 0])
+    worst_before, worst_after = np.inf, np.inf
+    for e in range(E):
+        I = po.triangle_info(1, ex[e], ey[e])
+        bf = np.concatenate([I["vol_bf"], I["edge_bf"]], axis=1)
+        for W, name in ((U, "before"), (V, "after")):
+            pts = np.einsum("vi,iq->vq", W[:, :, e], bf)
+            pr = 0.4 * (pts[3] - 0.5 * (pts[1] ** 2 + pts[2] ** 2) / pts[0])
+            m = min(pts[0].min(), pr.min()) if pts[0].min() > 0 else -1.0
+            if name == "before":
+                worst_before = min(worst_before, m)
+            else:
+                worst_after = min(worst_after, m)
+    assert worst_before < 0 < worst_after
+    assert (o.flags() & 2).sum() > 0 and (o.flags() & 1).sum() == 0
