@@ -227,6 +227,20 @@ const TriRule& projection_rule() {
     return R;
 }
 
+// psi_j at the points of the projection rule (element-independent), [q][j]
+const std::vector<double>& projection_psi(int P) {
+    static std::vector<double> tab[3];
+    static std::once_flag once[3];
+    std::call_once(once[P - 1], [P] {
+        const RefBasis& B = ref_basis(P);
+        const TriRule& R = projection_rule();
+        tab[P - 1].resize((size_t)R.n * B.Np);
+        for (int q = 0; q < R.n; ++q)
+            for (int j = 0; j < B.Np; ++j) tab[P - 1][(size_t)q * B.Np + j] = psi_value(B, j, R.xi[q], R.eta[q]);
+    });
+    return tab[P - 1];
+}
+
 void element_geometry(const double* x, const double* y, double* g) {
     double detJ = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0]);
     g[0] = (y[2] - y[0]) / detJ;    // xi_x
@@ -302,8 +316,8 @@ void element_to_native(int P, const double* x, const double* y, const double* c,
     const int Np = np_of(P);
     ElemBasis EB;
     element_basis(P, x, y, EB);
-    const RefBasis& B = ref_basis(P);
     const TriRule& R = projection_rule();
+    const std::vector<double>& ps = projection_psi(P);
     double num[NVAR][MAXNP] = {}, den[MAXNP] = {};
     for (int q = 0; q < R.n; ++q) {
         double px = x[0] + (x[1] - x[0]) * R.xi[q] + (x[2] - x[0]) * R.eta[q];
@@ -314,7 +328,7 @@ void element_to_native(int P, const double* x, const double* y, const double* c,
             for (int v = 0; v < NVAR; ++v) u[v] += c[v * Np + i] * f;
         }
         for (int j = 0; j < Np; ++j) {
-            double s = psi_value(B, j, R.xi[q], R.eta[q]);
+            double s = ps[(size_t)q * Np + j];
             for (int v = 0; v < NVAR; ++v) num[v][j] += R.w[q] * u[v] * s;
             den[j] += R.w[q] * s * s;
         }
@@ -327,15 +341,15 @@ void native_to_element(int P, const double* x, const double* y, const double* d,
     const int Np = np_of(P);
     ElemBasis EB;
     element_basis(P, x, y, EB);
-    const RefBasis& B = ref_basis(P);
     const TriRule& R = projection_rule();
+    const std::vector<double>& ps = projection_psi(P);
     double num[NVAR][MAXNP] = {}, den[MAXNP] = {};
     for (int q = 0; q < R.n; ++q) {
         double px = x[0] + (x[1] - x[0]) * R.xi[q] + (x[2] - x[0]) * R.eta[q];
         double py = y[0] + (y[1] - y[0]) * R.xi[q] + (y[2] - y[0]) * R.eta[q];
         double u[NVAR] = {0, 0, 0, 0};
         for (int j = 0; j < Np; ++j) {
-            double s = psi_value(B, j, R.xi[q], R.eta[q]);
+            double s = ps[(size_t)q * Np + j];
             for (int v = 0; v < NVAR; ++v) u[v] += d[v * Np + j] * s;
         }
         for (int i = 0; i < Np; ++i) {
@@ -387,8 +401,8 @@ void ic_state(int kind, const double* p, int np, double x, double y, double* U) 
 
 void project_native(int P, const double* x, const double* y, int kind, const double* p, int np, double* d) {
     const int Np = np_of(P);
-    const RefBasis& B = ref_basis(P);
     const TriRule& R = projection_rule();
+    const std::vector<double>& ps = projection_psi(P);
     double num[NVAR][MAXNP] = {}, den[MAXNP] = {};
     for (int q = 0; q < R.n; ++q) {
         double px = x[0] + (x[1] - x[0]) * R.xi[q] + (x[2] - x[0]) * R.eta[q];
@@ -396,7 +410,7 @@ void project_native(int P, const double* x, const double* y, int kind, const dou
         double U[NVAR];
         ic_state(kind, p, np, px, py, U);
         for (int j = 0; j < Np; ++j) {
-            double s = psi_value(B, j, R.xi[q], R.eta[q]);
+            double s = ps[(size_t)q * Np + j];
             for (int v = 0; v < NVAR; ++v) num[v][j] += R.w[q] * U[v] * s;
             den[j] += R.w[q] * s * s;
         }

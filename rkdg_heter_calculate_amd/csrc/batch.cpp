@@ -127,7 +127,8 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     const rkdg_params& p0 = params[0];
     if (p0.degree < 1 || p0.degree > 3) return bfail(nullptr, RKDG_ERR_ARG, "degree");
     if (p0.limiter != RKDG_LIMITER_NONE) return bfail(nullptr, RKDG_ERR_ARG, "the batching form runs without limiter");
-    if (p0.precision != RKDG_PREC_FP64) return bfail(nullptr, RKDG_ERR_ARG, "precision form not available in this build");
+    if (p0.precision != RKDG_PREC_FP64 && p0.precision != RKDG_PREC_FP32_MFMA)
+        return bfail(nullptr, RKDG_ERR_ARG, "precision: RKDG_PREC_FP64 or RKDG_PREC_FP32_MFMA");
     for (int i = 1; i < n; ++i)
         if (params[i].degree != p0.degree || params[i].flux != p0.flux || params[i].gamma != p0.gamma ||
             params[i].dt_policy != p0.dt_policy || params[i].precision != p0.precision)
@@ -138,6 +139,7 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     b->P = p0.degree;
     b->Np = np_of(b->P);
     b->block = p0.block > 0 ? p0.block : 256;
+    if (p0.precision == RKDG_PREC_FP32_MFMA) b->block = 64;  // 64 elements on 256 threads
     b->xcd_remap = 1;
     b->E.resize(n);
     b->off.resize(n);
@@ -393,6 +395,7 @@ int rkdg_batch_autotune(rkdg_batch* b, int32_t* best_block) {
     BCHK(b, hipEventCreate(&e1));
     for (int c : cands)
         for (int pw : pers) {
+            if (b->prm.precision == RKDG_PREC_FP32_MFMA && c != 64) continue;
             b->block = c;
             b->persistent = pw ? cus * pw * 256 / c : 0;
             int rc = upload_tiles(b);

@@ -202,7 +202,8 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
         return fail(nullptr, RKDG_ERR_ARG, "mesh arrays missing");
     if (prm->degree < 1 || prm->degree > 3) return fail(nullptr, RKDG_ERR_ARG, "degree must be 1, 2 or 3");
     if (prm->flux != RKDG_FLUX_LF && prm->flux != RKDG_FLUX_ROE) return fail(nullptr, RKDG_ERR_ARG, "flux");
-    if (prm->precision != RKDG_PREC_FP64) return fail(nullptr, RKDG_ERR_ARG, "precision form not available in this build");
+    if (prm->precision != RKDG_PREC_FP64 && prm->precision != RKDG_PREC_FP32_MFMA)
+        return fail(nullptr, RKDG_ERR_ARG, "precision: RKDG_PREC_FP64 or RKDG_PREC_FP32_MFMA");
     if (!(prm->cfl > 0) && prm->dt_policy != RKDG_DT_FIXED) return fail(nullptr, RKDG_ERR_ARG, "cfl");
     if (prm->dt_policy == RKDG_DT_FIXED && !(prm->dt_fixed > 0)) return fail(nullptr, RKDG_ERR_ARG, "dt_fixed");
     if (m->n_elements > (1 << 29) - 1) return fail(nullptr, RKDG_ERR_ARG, "too many elements");
@@ -211,6 +212,7 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     c->P = prm->degree; c->Np = np_of(c->P); c->Nqv = nqv_of(c->P); c->Nqe = nqe_of(c->P);
     c->E = m->n_elements;
     c->block = prm->block > 0 ? prm->block : env_int("RKDG_BLOCK", 256);
+    if (prm->precision == RKDG_PREC_FP32_MFMA) c->block = 64;  // 64 elements on 256 threads
     if (c->block % 64 || c->block < 64 || c->block > 256) { delete c; return fail(nullptr, RKDG_ERR_ARG, "block must be 64..256, multiple of 64"); }
     c->xcd_remap = env_int("RKDG_XCD_REMAP", 1);
     c->ntiles = (c->E + c->block - 1) / c->block;
@@ -709,11 +711,8 @@ int rkdg_error_vs_exact(rkdg_ctx* c, int32_t kind, const double* p, int32_t np, 
     std::vector<double> U((size_t)4 * Np * E);
     int rc = rkdg_get_state(c, RKDG_BASIS_NATIVE, U.data());
     if (rc) return rc;
-    const RefBasis& B = ref_basis(P);
     const TriRule& R = projection_rule();
-    std::vector<double> psi((size_t)R.n * Np);
-    for (int q = 0; q < R.n; ++q)
-        for (int i = 0; i < Np; ++i) psi[(size_t)q * Np + i] = psi_value(B, i, R.xi[q], R.eta[q]);
+    const std::vector<double>& psi = projection_psi(P);
     const int nt = 16;
     std::vector<double> acc((size_t)nt * 12, 0.0);
     std::vector<int> chunk_id(1, 0);

@@ -65,6 +65,7 @@ void psi_grad(const RefBasis& B, int i, double xi, double eta, double* gx, doubl
 // collapsed 6x6 Gauss rule on the reference triangle; weights sum to 1
 struct TriRule { int n; std::vector<double> xi, eta, w; };
 const TriRule& projection_rule();
+const std::vector<double>& projection_psi(int P);  // psi_j at the rule's points, [q][j]
 
 // Element geometry rows for one element (GEO_ROWS doubles)
 void element_geometry(const double* x, const double* y, double* g);

@@ -435,6 +435,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 
     }
 }
 
+#include "stage_mx.inc"
+
 // CFL wave speed of the current state, max-reduced into ws[slot]
 template <int P>
 __global__ void __launch_bounds__(256) k_wavespeed(StageArgs a) {
@@ -634,9 +636,26 @@ static void launch_stage_p(int flux, int block, const StageArgs& a, hipStream_t 
     else launch_stage_pf<P, RKDG_FLUX_LF>(block, a, s);
 }
 
+template <int P>
+static void launch_stage_mx_p(int flux, const StageArgs& a, hipStream_t s) {
+    dim3 grid(grid_blocks(a)), blk(256);
+    if (flux == RKDG_FLUX_ROE) hipLaunchKernelGGL((k_stage_mx<P, RKDG_FLUX_ROE>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_stage_mx<P, RKDG_FLUX_LF>), grid, blk, 0, s, a);
+}
+
 int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream) {
-    (void)prec;
     hipStream_t s = (hipStream_t)stream;
+    if (prec == RKDG_PREC_FP32_MFMA) {
+        // four lanes per element: tiles of 64 elements on 256 threads
+        if (block != 64) return (int)hipErrorInvalidValue;
+        switch (P) {
+            case 1: launch_stage_mx_p<1>(flux, a, s); break;
+            case 2: launch_stage_mx_p<2>(flux, a, s); break;
+            case 3: launch_stage_mx_p<3>(flux, a, s); break;
+            default: return (int)hipErrorInvalidValue;
+        }
+        return (int)hipGetLastError();
+    }
     switch (P) {
         case 1: launch_stage_p<1>(flux, block, a, s); break;
         case 2: launch_stage_p<2>(flux, block, a, s); break;

@@ -225,3 +225,81 @@ def test_solver_driver_runs_reference_case(tmp_path):
                                      L.dptr(U)), None, "load")
     Us = s.get_state()
     assert np.allclose(U, Us, rtol=1e-13, atol=1e-14)
+
+
+# ---- precision form: FP32 MFMA contractions, FP64 state / means / update ----------------
+MX_TOL = 1e-6  # relative L2 per variable against the FP64 oracle
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+@pytest.mark.parametrize("flux", ["lf", "roe"])
+def test_precision_form_single_stage(P, flux):
+    print(f"case: FP32-MFMA single stage P{P} {flux}", flush=True)
+    g = vortex_grid(8, jitter=0.2, diagonal=1, seed=3)
+    o, _ = make_pair(g, P=P, flux=flux)
+    U0 = vortex_ic(o)
+    s = Solver(g, degree=P, flux=flux, dt_policy="max", cfl=0.3, precision="fp32")
+    s.set_state(U0)
+    for st in range(3):
+        o.stage(st, 0.01)
+        s.stage(st, 0.01)
+        err = rel_l2(s.get_state(), o.get_state(), o.mass())
+        assert np.all(err < MX_TOL), (st, err)
+        assert np.all(err > 0)  # really computed in reduced precision
+
+
+def test_precision_form_vortex_steps():
+    print("case: FP32-MFMA 50 steps P2", flush=True)
+    g = vortex_grid(32, jitter=0.2, diagonal=1, seed=5)
+    o, _ = make_pair(g, P=2, flux="lf", dt_policy="max", cfl=0.3)
+    s = Solver(g, degree=2, flux="lf", dt_policy="max", cfl=0.3, precision="fp32")
+    s.set_state(vortex_ic(o))
+    s.advance(50)
+    o.advance(50)
+    err = rel_l2(s.get_state(), o.get_state(), o.mass())
+    print("relative L2 vs FP64 oracle", err)
+    assert np.all(err < MX_TOL), err
+
+
+def test_precision_form_boundaries_and_limiter():
+    print("case: FP32-MFMA boundaries + limiter", flush=True)
+    g = Grid.splitquad(24, 24, 0.0, 0.0, 1.0, 1.0, periodic=False, diagonal=1, jitter=0.2, seed=9)
+    o, _ = make_pair(g, P=1, flux="roe", limiter=True, dt_policy="max", cfl=0.2)
+    o.project(2, riemann4_params())
+    s = Solver(g, degree=1, flux="roe", limiter=True, dt_policy="max", cfl=0.2, precision="fp32")
+    s.set_state(o.get_state())
+    s.advance(5)
+    o.advance(5)
+    err = rel_l2(s.get_state(), o.get_state(), o.mass())
+    assert np.all(err < 1e-5), err
+
+
+def test_precision_form_batch_equals_single():
+    print("case: FP32-MFMA batch vs single", flush=True)
+    import ctypes
+    from rkdg_heter_calculate_amd import _lib as L
+    from rkdg_heter_calculate_amd.solver import make_params, mesh_view
+    lib = L.load()
+    grids = [vortex_grid(n, jitter=0.2, diagonal=1, seed=n) for n in (7, 12)]
+    meshes = (L.Mesh * 2)()
+    views = [mesh_view(gr) for gr in grids]
+    for i, v in enumerate(views):
+        meshes[i] = v
+    params = (L.Params * 2)()
+    for i in range(2):
+        params[i] = make_params(degree=2, dt_policy="max", cfl=0.3, precision="fp32")
+    h = ctypes.c_void_p()
+    L.check(lib.rkdg_batch_create(2, meshes, params, ctypes.byref(h)), None, "batch_create")
+    vp = np.array(vortex_params(), dtype=np.float64)
+    for i in range(2):
+        L.check(lib.rkdg_batch_project_ic(h, i, 1, L.dptr(vp), len(vp)), None, "project")
+    L.check(lib.rkdg_batch_advance(h, 6, 1), None, "advance")
+    L.check(lib.rkdg_batch_sync(h), None, "sync")
+    for i, gr in enumerate(grids):
+        s = Solver(gr, degree=2, dt_policy="max", cfl=0.3, precision="fp32")
+        s.project_ic(1, vortex_params())
+        s.advance(6)
+        Ub = np.empty((4, 6, gr.n_elements))
+        L.check(lib.rkdg_batch_get_state(h, i, 1, L.dptr(Ub)), None, "get")
+        assert np.array_equal(Ub, s.get_state(basis=1)), i
+    lib.rkdg_batch_destroy(h)
