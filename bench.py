@@ -133,6 +133,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 2.1M-triangle and batched legs")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batched leg")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="size of the stream-copy measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the hot path on one GPU; --gpus {args.gpus} is not supported "
@@ -172,6 +173,14 @@ def main():
                "sample": f"{args.cpu_steps} SSP-RK3 steps of the same {E}-triangle P{P} mesh "
                          f"({secs:.1f} s, OpenMP over elements)"}
 
+    copy_gbs = None
+    if args.copy_mib > 0:
+        import ctypes
+        from rkdg_heter_calculate_amd import _lib as L
+        gbs = ctypes.c_double()
+        L.check(L.load().rkdg_measure_copy(0, args.copy_mib << 20, 20, ctypes.byref(gbs)), None, "measure_copy")
+        copy_gbs = gbs.value
+
     extra = {}
     if not args.no_extra:
         extra["p2_2M_fp64"] = large_p2_point(1024, 20)
@@ -196,7 +205,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"k_stage<{P},LF>", "bytes_per_launch": per_launch,
-                     "avg_launch_ms": r["stage_ms"]},
+                     "avg_launch_ms": r["stage_ms"], "measured_copy_GBs": copy_gbs},
         "cpu_baseline": cpu,
         "wall_s": wall,
         "l2_vs_exact": err["l2"],

@@ -161,6 +161,12 @@ int rkdg_time_advance(rkdg_ctx* ctx, int32_t nsteps, double* total_ms);
  * out[0..3] relative L2 per variable, out[4..7] max abs error at the rule's points. */
 int rkdg_error_vs_exact(rkdg_ctx* ctx, int32_t kind, const double* p, int32_t np, double out[8]);
 
+/* Measurement helper: times `reps` launches of a plain device-to-device copy kernel
+ * (k_stream_copy) of `bytes` bytes (read once, written once) on `device`; returns the
+ * copy bandwidth (bytes read + written per second, GB/s).  Used to quote the measured
+ * HBM bandwidth next to the vendor peak and to calibrate memory counters. */
+int rkdg_measure_copy(int32_t device, int64_t bytes, int32_t reps, double* gbs);
+
 /* Sizes the library derives: Np, Nqv, Nqe, n_elements, device bytes. */
 int rkdg_info(rkdg_ctx* ctx, int64_t out[5]);
 

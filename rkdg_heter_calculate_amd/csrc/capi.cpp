@@ -754,6 +754,36 @@ int rkdg_error_vs_exact(rkdg_ctx* c, int32_t kind, const double* p, int32_t np, 
     return RKDG_OK;
 }
 
+int rkdg_measure_copy(int32_t device, int64_t bytes, int32_t reps, double* gbs) {
+    if (bytes < 16 || reps < 1 || !gbs) return fail(nullptr, RKDG_ERR_ARG, "measure_copy");
+    bytes = bytes / 16 * 16;
+    HIPCHK(nullptr, hipSetDevice(device));
+    hipStream_t s;
+    HIPCHK(nullptr, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    void *a = nullptr, *b = nullptr;
+    hipError_t e1 = hipMalloc(&a, bytes), e2 = hipMalloc(&b, bytes);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        hipFree(a); hipFree(b); hipStreamDestroy(s);
+        return fail(nullptr, RKDG_ERR_NOMEM, "measure_copy buffers");
+    }
+    hipMemsetAsync(a, 0, bytes, s);
+    hipEvent_t e0, e9;
+    hipEventCreate(&e0);
+    hipEventCreate(&e9);
+    launch_stream_copy(a, b, bytes, s);  // warm-up
+    hipEventRecord(e0, s);
+    for (int r = 0; r < reps; ++r) launch_stream_copy(a, b, bytes, s);
+    hipEventRecord(e9, s);
+    hipError_t st = hipEventSynchronize(e9);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e9);
+    hipEventDestroy(e0); hipEventDestroy(e9);
+    hipFree(a); hipFree(b); hipStreamDestroy(s);
+    if (st != hipSuccess) return fail(nullptr, RKDG_ERR_HIP, hipGetErrorString(st));
+    *gbs = 2.0 * (double)bytes * reps / (ms / 1e3) / 1e9;
+    return RKDG_OK;
+}
+
 int rkdg_info(rkdg_ctx* c, int64_t out[5]) {
     if (!c || !out) return RKDG_ERR_ARG;
     out[0] = c->Np; out[1] = c->Nqv; out[2] = c->Nqe; out[3] = c->E; out[4] = (int64_t)c->bytes;

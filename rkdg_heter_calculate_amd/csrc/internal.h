@@ -148,6 +148,7 @@ int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void*
 int launch_wavespeed(int P, const StageArgs& a, int block, void* stream);
 int launch_state_init(DevState* st, int slot, double t, int n, void* stream);
 int launch_limiter(int P, const LimArgs& a, int block, void* stream);
+int launch_stream_copy(const void* src, void* dst, long long bytes, void* stream);
 const char* hip_error_string(int code);
 
 }  // namespace rkdg

@@ -600,6 +600,22 @@ __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
     a.flags[e] = flag;
 }
 
+// Plain streaming copy (16 bytes per lane per iteration), the calibration kernel for
+// bandwidth and memory counters.
+__global__ void __launch_bounds__(256) k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                     long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+int launch_stream_copy(const void* src, void* dst, long long bytes, void* stream) {
+    const long long n = bytes / 16;
+    hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, (hipStream_t)stream, (const double2*)src,
+                       (double2*)dst, n);
+    return (int)hipGetLastError();
+}
+
 int upload_tables(int P, const DevTables& t) {
     hipError_t rc = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof(DevTables), (size_t)(P - 1) * sizeof(DevTables),
                                       hipMemcpyHostToDevice);

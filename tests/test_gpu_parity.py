@@ -180,7 +180,8 @@ def test_naca_reference_case_matches_oracle(tmp_path):
     ic = [1.4, 5.0 * np.arctan(1.0) * 4 / 180, 0.4, 1.0, 1.0]
     o.project(0, ic)
     s.project_ic(IC_FREESTREAM, ic)
-    assert np.array_equal(s.get_state(), o.get_state())
+    # the exchange through the element basis is an L2 projection: round-off in the modes
+    assert np.allclose(s.get_state(), o.get_state(), rtol=0, atol=1e-13)
     for _ in range(11):
         dto = o.compute_dt()
         assert abs(s.step() - dto) <= 1e-12 * dto
