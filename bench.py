@@ -91,6 +91,50 @@ def batch_point(count, steps):
             "note": "FP64 batched form, hipGraph-replayed stages, autotuned workgroup size"}
 
 
+def precision_point(g, P, steps):
+    """FP32-MFMA form on the same workload: timing and its error against the FP64 form.
+
+    The error runs use one fixed dt (the FP64 form's first CFL dt) so that both forms take
+    identical steps; the FP64 form matches the oracle to 1e-12, so the difference is the
+    precision form's own error."""
+    import numpy as np
+    from rkdg_heter_calculate_amd import Solver, IC_VORTEX, vortex_params
+    from rkdg_heter_calculate_amd import _lib as L
+    s = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=0.3, precision="fp32")
+    s.project_ic(IC_VORTEX, vortex_params())
+    s.advance(10)
+    r = s.time_advance(steps)
+    t_sim, _ = s.time()
+    err_exact = s.error_vs_exact(IC_VORTEX, vortex_params(t=t_sim))
+    s.close()
+    E = g.n_elements
+    Np = (P + 1) * (P + 2) // 2
+    out = {"value": E * Np * 4 * 3 * steps / (r["total_ms"] / 1e3), "ms_per_step": r["total_ms"] / steps,
+           "stage_ms": r["stage_ms"], "l2_vs_exact": err_exact["l2"], "linf_vs_exact": err_exact["linf"]}
+    # matched-step comparison against the FP64 form
+    d = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=0.3)
+    d.project_ic(IC_VORTEX, vortex_params())
+    dt0 = d.compute_dt()
+    d.close()
+    res = {}
+    for prec in ("fp64", "fp32"):
+        x = Solver(g, degree=P, flux="lf", dt_policy="fixed", dt_fixed=dt0, precision=prec)
+        x.project_ic(IC_VORTEX, vortex_params())
+        x.advance(steps)
+        res[prec] = x.get_state(L.BASIS_NATIVE)  # orthonormal on the reference triangle
+        x.close()
+    a, b = res["fp64"], res["fp32"]
+    ex, ey = g.element_coords()
+    area = 0.5 * np.abs((ex[:, 1] - ex[:, 0]) * (ey[:, 2] - ey[:, 0]) - (ex[:, 2] - ex[:, 0]) * (ey[:, 1] - ey[:, 0]))
+    # L2 norm of a field = sqrt(sum_e |J_e| sum_i c_i^2) in the orthonormal reference basis
+    rel = [float(np.sqrt(np.sum(area * np.sum((b[v] - a[v]) ** 2, axis=0)) / np.sum(area * np.sum(a[v] ** 2, axis=0))))
+           for v in range(4)]
+    linf = [float(np.max(np.abs(b[v] - a[v])) / np.max(np.abs(a[v]))) for v in range(4)]
+    out.update({"rel_l2_vs_fp64": rel, "rel_linf_vs_fp64": linf, "compare_steps": steps, "compare_dt": dt0,
+                "note": "FP32 contractions on v_mfma_f32_4x4x1f32, FP64 state, means, alpha, RK and CFL"})
+    return out
+
+
 def cpu_baseline(g, P, steps, threads):
     """The oracle (CPU restatement of the reference algorithm) on a bounded sample."""
     from oracle import pyoracle as po
@@ -129,9 +173,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512, help="split quads per side")
     ap.add_argument("--degree", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=50, help="oracle steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 2.1M-triangle and batched legs")
+    ap.add_argument("--no-precision", action="store_true", help="skip the FP32-MFMA leg")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batched leg")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the stream-copy measurement")
     args = ap.parse_args()
@@ -181,6 +226,10 @@ def main():
         L.check(L.load().rkdg_measure_copy(0, args.copy_mib << 20, 20, ctypes.byref(gbs)), None, "measure_copy")
         copy_gbs = gbs.value
 
+    prec = None
+    if not args.no_precision:
+        prec = precision_point(g, P, args.steps)
+
     extra = {}
     if not args.no_extra:
         extra["p2_2M_fp64"] = large_p2_point(1024, 20)
@@ -207,6 +256,7 @@ def main():
                      "kernel": f"k_stage<{P},LF>", "bytes_per_launch": per_launch,
                      "avg_launch_ms": r["stage_ms"], "measured_copy_GBs": copy_gbs},
         "cpu_baseline": cpu,
+        "precision_form": prec,
         "wall_s": wall,
         "l2_vs_exact": err["l2"],
         "linf_vs_exact": err["linf"],

@@ -8,7 +8,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "librkdg.so")
+# RKDG_LIB selects another build of the library (e.g. one compiled with other flags)
+LIB_PATH = os.environ.get("RKDG_LIB") or os.path.join(_HERE, "librkdg.so")
 
 c_int32 = ctypes.c_int32
 c_int64 = ctypes.c_int64

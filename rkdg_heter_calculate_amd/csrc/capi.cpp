@@ -39,6 +39,51 @@ void parallel_for(int n, const std::function<void(int, int)>& body) {
     for (auto& x : th) x.join();
 }
 
+int build_halo(const int* nbr, long long ld, int E, int B, std::vector<int>& hoff, std::vector<int>& hids,
+               std::vector<int>& nloc) {
+    const int ntiles = (E + B - 1) / B;
+    hoff.assign(ntiles + 1, 0);
+    hids.clear();
+    nloc.assign((size_t)3 * ld, 0);
+    int hmax = 0;
+    std::vector<int> h;
+    for (int t = 0; t < ntiles; ++t) {
+        const int lo = t * B, hi = std::min(E, lo + B);
+        h.clear();
+        for (int k = 0; k < 3; ++k)
+            for (int p = lo; p < hi; ++p) {
+                const int w = nbr[(size_t)k * ld + p];
+                if (w >= 0 && ((w >> 2) < lo || (w >> 2) >= hi)) h.push_back(w >> 2);
+            }
+        std::sort(h.begin(), h.end());
+        h.erase(std::unique(h.begin(), h.end()), h.end());
+        for (int k = 0; k < 3; ++k)
+            for (int p = lo; p < hi; ++p) {
+                const int w = nbr[(size_t)k * ld + p];
+                int v = w;
+                if (w >= 0) {
+                    const int n = w >> 2;
+                    const int slot = (n >= lo && n < hi) ? n - lo
+                                                         : B + (int)(std::lower_bound(h.begin(), h.end(), n) - h.begin());
+                    v = (slot << 2) | (w & 3);
+                }
+                nloc[(size_t)k * ld + p] = v;
+            }
+        hids.insert(hids.end(), h.begin(), h.end());
+        hoff[t + 1] = (int)hids.size();
+        hmax = std::max(hmax, (int)h.size());
+    }
+    for (int k = 0; k < 3; ++k)
+        for (long long p = E; p < ld; ++p) nloc[(size_t)k * ld + p] = nbr[(size_t)k * ld + p];
+    return hmax;
+}
+
+int max_lds_per_block(int device) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return 65536;
+    return v;
+}
+
 }  // namespace rkdg
 
 using namespace rkdg;
@@ -54,6 +99,11 @@ struct rkdg_ctx {
     double* U[3] = {nullptr, nullptr, nullptr};
     double* geo = nullptr;
     int* nbr = nullptr;
+    int* nloc = nullptr;        // staged form: neighbours as LDS slots
+    int* hoff = nullptr;        // staged form: per-tile halo offsets
+    int* hids = nullptr;        // staged form: halo element positions
+    int lds_stride = 0;         // staged form: doubles per LDS row (block + largest halo)
+    int staged = 0;
     unsigned char* dtflag = nullptr;
     double* lim = nullptr;
     int* pairs = nullptr;
@@ -96,6 +146,11 @@ static StageArgs make_args(rkdg_ctx* c, int stage, int slot, int dt_mode, double
     a.Uout = c->U[u_out];
     a.geo = c->geo;
     a.nbr = c->nbr;
+    a.nloc = c->nloc;
+    a.hoff = c->hoff;
+    a.hids = c->hids;
+    a.lds_stride = c->lds_stride;
+    a.staged = c->staged;
     a.dtflag = c->dtflag;
     a.st = c->st;
     a.ld = c->ld;
@@ -211,7 +266,7 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     c->prm = *prm;
     c->P = prm->degree; c->Np = np_of(c->P); c->Nqv = nqv_of(c->P); c->Nqe = nqe_of(c->P);
     c->E = m->n_elements;
-    c->block = prm->block > 0 ? prm->block : env_int("RKDG_BLOCK", 256);
+    c->block = prm->block > 0 ? prm->block : env_int("RKDG_BLOCK", prm->degree >= 3 ? 128 : 256);
     if (prm->precision == RKDG_PREC_FP32_MFMA) c->block = 64;  // 64 elements on 256 threads
     if (c->block % 64 || c->block < 64 || c->block > 256) { delete c; return fail(nullptr, RKDG_ERR_ARG, "block must be 64..256, multiple of 64"); }
     c->xcd_remap = env_int("RKDG_XCD_REMAP", 1);
@@ -223,7 +278,27 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     // locality permutation
     c->order.resize(E);
     c->pos.resize(E);
-    for (int p = 0; p < E; ++p) c->order[p] = m->order ? m->order[p] : p;
+    if (m->order) {
+        for (int p = 0; p < E; ++p) c->order[p] = m->order[p];
+    } else if (env_int("RKDG_IDENTITY_ORDER", 0)) {
+        for (int p = 0; p < E; ++p) c->order[p] = p;
+    } else {
+        // Z-order of the barycentres: workgroup tiles become compact patches, so most
+        // neighbours of a tile lie inside it (small halos) and the rest are nearby
+        std::vector<double> bx(E), by(E);
+        for (int e = 0; e < E; ++e) {
+            double sx = 0, sy = 0;
+            for (int k = 0; k < 3; ++k) {
+                const int v = m->tri[3 * e + k];
+                if (v < 0 || v >= m->n_vertices) { delete c; return fail(nullptr, RKDG_ERR_MESH, "vertex id out of range"); }
+                sx += m->vx[v];
+                sy += m->vy[v];
+            }
+            bx[e] = sx / 3;
+            by[e] = sy / 3;
+        }
+        morton_order(E, bx.data(), by.data(), c->order.data());
+    }
     std::vector<char> seen(E, 0);
     for (int p = 0; p < E; ++p) {
         int id = c->order[p];
@@ -322,6 +397,20 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
         }
     }
 
+    // staged form: per-tile halos, used when the tile's LDS image fits
+    std::vector<int> hoff, hids, hnloc;
+    if (prm->precision == RKDG_PREC_FP64 && env_int("RKDG_STAGED", 1)) {
+        const int hmax = build_halo(hnbr.data(), ld, E, c->block, hoff, hids, hnloc);
+        const int stride = c->block + hmax;
+        const long long lds = (long long)4 * c->Np * stride * (long long)sizeof(double);
+        const long long cap = std::min<long long>(max_lds_per_block(prm->device),
+                                                  (long long)env_int("RKDG_STAGED_MAX_KB", 80) * 1024);
+        if (lds <= cap) {
+            c->staged = 1;
+            c->lds_stride = stride;
+        }
+    }
+
     // device
     c->dev = prm->device;
     HIPCHK(c, hipSetDevice(c->dev));
@@ -339,6 +428,16 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     HIPCHK(c, hipMemcpyAsync(c->geo, hgeo.data(), sizeof(double) * hgeo.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->nbr, hnbr.data(), sizeof(int) * hnbr.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice, c->stream));
+    if (c->staged) {
+        HIPCHK(c, hipMalloc(&c->nloc, sizeof(int) * hnloc.size()));
+        HIPCHK(c, hipMalloc(&c->hoff, sizeof(int) * hoff.size()));
+        HIPCHK(c, hipMalloc(&c->hids, sizeof(int) * std::max<size_t>(1, hids.size())));
+        c->bytes += sizeof(int) * (hnloc.size() + hoff.size() + hids.size());
+        HIPCHK(c, hipMemcpyAsync(c->nloc, hnloc.data(), sizeof(int) * hnloc.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->hoff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice, c->stream));
+        if (!hids.empty())
+            HIPCHK(c, hipMemcpyAsync(c->hids, hids.data(), sizeof(int) * hids.size(), hipMemcpyHostToDevice, c->stream));
+    }
     if (prm->limiter == RKDG_LIMITER_TVB) {
         HIPCHK(c, hipMalloc(&c->lim, sizeof(double) * hlim.size()));
         HIPCHK(c, hipMalloc(&c->pairs, sizeof(int) * hpairs.size()));
@@ -362,6 +461,7 @@ int rkdg_destroy(rkdg_ctx* c) {
     if (c->graph2) hipGraphExecDestroy(c->graph2);
     for (int b = 0; b < 3; ++b) hipFree(c->U[b]);
     hipFree(c->geo); hipFree(c->nbr); hipFree(c->dtflag); hipFree(c->lim); hipFree(c->pairs);
+    hipFree(c->nloc); hipFree(c->hoff); hipFree(c->hids);
     hipFree(c->flags); hipFree(c->st);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

@@ -126,6 +126,28 @@ using namespace rkdg;
 
 static thread_local std::string g_grid_err;
 
+namespace rkdg {
+
+void morton_order(int n, const double* bx, const double* by, int32_t* order) {
+    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+    for (int i = 0; i < n; ++i) {
+        xmin = std::min(xmin, bx[i]); xmax = std::max(xmax, bx[i]);
+        ymin = std::min(ymin, by[i]); ymax = std::max(ymax, by[i]);
+    }
+    std::vector<std::pair<uint64_t, int32_t>> key(n);
+    for (int i = 0; i < n; ++i) {
+        uint32_t ix = (uint32_t)std::min(65535.0, std::max(0.0, (bx[i] - xmin) / (xmax - xmin + 1e-300) * 65535.0));
+        uint32_t iy = (uint32_t)std::min(65535.0, std::max(0.0, (by[i] - ymin) / (ymax - ymin + 1e-300) * 65535.0));
+        uint64_t m = 0;
+        for (int b = 0; b < 16; ++b) m |= (uint64_t)((ix >> b) & 1) << (2 * b) | (uint64_t)((iy >> b) & 1) << (2 * b + 1);
+        key[i] = {m, i};
+    }
+    std::stable_sort(key.begin(), key.end());
+    for (int i = 0; i < n; ++i) order[i] = key[i].second;
+}
+
+}  // namespace rkdg
+
 extern "C" {
 
 int rkdg_grid_splitquad(int32_t nx, int32_t ny, double x0, double y0, double lx, double ly, int32_t periodic,
@@ -229,25 +251,22 @@ int rkdg_grid_read(const char* meshconf, const char* partition_file, double box,
 int rkdg_grid_locality_order(const rkdg_grid* g, int32_t* order) {
     if (!g || !order) return RKDG_ERR_ARG;
     const int E = g->n_elements;
-    std::vector<std::pair<uint64_t, int32_t>> key(E);
-    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
-    for (int i = 0; i < g->n_vertices; ++i) {
-        xmin = std::min(xmin, g->vx[i]); xmax = std::max(xmax, g->vx[i]);
-        ymin = std::min(ymin, g->vy[i]); ymax = std::max(ymax, g->vy[i]);
+    if (g->partition) {
+        // gridpartition's parts, each in element order
+        std::vector<std::pair<uint64_t, int32_t>> key(E);
+        for (int e = 0; e < E; ++e) key[e] = {((uint64_t)(uint32_t)g->partition[e] << 32) | (uint64_t)e, e};
+        std::stable_sort(key.begin(), key.end());
+        for (int i = 0; i < E; ++i) order[i] = key[i].second;
+        return RKDG_OK;
     }
+    std::vector<double> bx(E), by(E);
     for (int e = 0; e < E; ++e) {
-        double bx = 0, by = 0;
-        for (int k = 0; k < 3; ++k) { bx += g->vx[g->tri[3 * e + k]]; by += g->vy[g->tri[3 * e + k]]; }
-        bx /= 3; by /= 3;
-        uint32_t ix = (uint32_t)std::min(65535.0, std::max(0.0, (bx - xmin) / (xmax - xmin + 1e-300) * 65535.0));
-        uint32_t iy = (uint32_t)std::min(65535.0, std::max(0.0, (by - ymin) / (ymax - ymin + 1e-300) * 65535.0));
-        uint64_t m = 0;
-        for (int b = 0; b < 16; ++b) m |= (uint64_t)((ix >> b) & 1) << (2 * b) | (uint64_t)((iy >> b) & 1) << (2 * b + 1);
-        uint64_t p = g->partition ? (uint64_t)(uint32_t)g->partition[e] : 0;
-        key[e] = {g->partition ? (p << 32) | (uint64_t)e : m, e};
+        double sx = 0, sy = 0;
+        for (int k = 0; k < 3; ++k) { sx += g->vx[g->tri[3 * e + k]]; sy += g->vy[g->tri[3 * e + k]]; }
+        bx[e] = sx / 3;
+        by[e] = sy / 3;
     }
-    std::stable_sort(key.begin(), key.end());
-    for (int i = 0; i < E; ++i) order[i] = key[i].second;
+    rkdg::morton_order(E, bx.data(), by.data(), order);
     return RKDG_OK;
 }
 

@@ -96,6 +96,16 @@ void limiter_geometry(const double* x, const double* y, const double nbx[3], con
 
 // Mesh helpers (grid.cpp)
 int build_shared_edges(int E, const int32_t* tri, const int32_t* nbr, int32_t* se, std::string& err);
+// Morton (Z-order) permutation of points (bx, by): order[p] = index of the p-th point.
+void morton_order(int n, const double* bx, const double* by, int32_t* order);
+// Staged-form tables for tiles of B consecutive device positions: per tile the sorted
+// out-of-tile neighbour positions (hids[hoff[t] .. hoff[t+1])) and each element's
+// neighbours re-encoded as slots of the tile's LDS image (slot < B: own tile, else
+// B + index into the tile's halo).  nbr is [3][ld] packed (pos << 2) | edge or -tag.
+// Returns the largest halo.
+int build_halo(const int* nbr, long long ld, int E, int B, std::vector<int>& hoff, std::vector<int>& hids,
+               std::vector<int>& nloc);
+int max_lds_per_block(int device);
 
 // Config parser (io.cpp)
 bool config_parse(const char* path, std::vector<std::pair<std::string, std::string>>& items);
@@ -127,6 +137,12 @@ struct StageArgs {
     double dt_host;
     double cfl, gamma, t_end;
     double fs[4];
+    // staged form: neighbour slots in the tile's LDS image and the per-tile halo lists
+    const int* nloc;       // [3][ld] packed (slot << 2) | shared edge, or -tag for a boundary
+    const int* hoff;       // [ntiles + 1] offsets into hids
+    const int* hids;       // halo element positions, slot blockDim.x + h of tile t is hids[hoff[t] + h]
+    int lds_stride;        // >= block + largest halo
+    int staged;
 };
 
 enum { DTM_DEVICE = 0, DTM_HOST = 1, DTM_FIXED = 2 };

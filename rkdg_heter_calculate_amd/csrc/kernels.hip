@@ -104,45 +104,48 @@ __device__ __forceinline__ double pressure(double g, const double* U, double ir)
 }
 
 // Roe (1981) normal flux with the Harten-Hyman (1983) entropy fix on the acoustic waves.
+// The central part 0.5 (F(UL) + F(UR)).n is formed from the two directional fluxes
+// directly, so only four partial sums stay live instead of both F and G of both states.
 __device__ __forceinline__ void roe_flux(double g, const double* UL, const double* UR, double nx, double ny,
                                          double* Fh) {
-    double FL[4], GL[4], FR[4], GR[4];
-    euler_fg(g, UL, FL, GL);
-    euler_fg(g, UR, FR, GR);
-    const double rl = UL[0], ul = UL[1] / rl, vl = UL[2] / rl;
+    const double rl = UL[0], irl = frcp(rl), ul = UL[1] * irl, vl = UL[2] * irl;
+    const double rr = UR[0], irr = frcp(rr), ur = UR[1] * irr, vr = UR[2] * irr;
     const double pl = (g - 1) * (UL[3] - 0.5 * rl * (ul * ul + vl * vl));
-    const double rr = UR[0], ur = UR[1] / rr, vr = UR[2] / rr;
     const double pr = (g - 1) * (UR[3] - 0.5 * rr * (ur * ur + vr * vr));
-    const double hl = (UL[3] + pl) / rl, hr = (UR[3] + pr) / rr;
-    const double sl = sqrt(rl), sr = sqrt(rr), isum = 1.0 / (sl + sr);
+    double FL[4], FR[4];
+    euler_dir(g, UL, irl, pl, nx, ny, FL);
+    euler_dir(g, UR, irr, pr, nx, ny, FR);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fh[v] = FL[v] + FR[v];
+    const double hl = (UL[3] + pl) * irl, hr = (UR[3] + pr) * irr;
+    const double sl = sqrt(rl), sr = sqrt(rr), isum = frcp(sl + sr);
     const double rt = sl * sr;
     const double ut = (sl * ul + sr * ur) * isum, vt = (sl * vl + sr * vr) * isum, ht = (sl * hl + sr * hr) * isum;
     const double q2 = ut * ut + vt * vt;
     const double c2 = (g - 1) * (ht - 0.5 * q2);
-    const double ct = sqrt(c2);
+    const double ct = sqrt(c2), ic2 = frcp(c2);
     const double unt = ut * nx + vt * ny, utt = -ut * ny + vt * nx;
     const double dp = pr - pl, dr = rr - rl;
-    const double dun = (ur * nx + vr * ny) - (ul * nx + vl * ny);
-    const double dut = (-ur * ny + vr * nx) - (-ul * ny + vl * nx);
-    const double a1 = (dp - rt * ct * dun) / (2 * c2);
-    const double a2 = dr - dp / c2;
-    const double a3 = rt * dut;
-    const double a4 = (dp + rt * ct * dun) / (2 * c2);
-    // Harten-Hyman entropy fix on the acoustic waves (vanishes across shocks)
-    const double cl = sqrt(g * pl / rl), cr = sqrt(g * pr / rr);
     const double unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
-    double l1 = fabs(unt - ct), l2 = fabs(unt), l4 = fabs(unt + ct);
+    const double dun = unr - unl;
+    const double dut = (-ur * ny + vr * nx) - (-ul * ny + vl * nx);
+    const double a1 = 0.5 * (dp - rt * ct * dun) * ic2;
+    const double a2 = dr - dp * ic2;
+    const double a3 = rt * dut;
+    const double a4 = 0.5 * (dp + rt * ct * dun) * ic2;
+    // Harten-Hyman entropy fix on the acoustic waves (vanishes across shocks)
+    const double cl = sqrt(g * pl * irl), cr = sqrt(g * pr * irr);
+    double l1 = fabs(unt - ct), l4 = fabs(unt + ct);
+    const double l2 = fabs(unt);
     const double d1 = fmax(0.0, fmax((unt - ct) - (unl - cl), (unr - cr) - (unt - ct)));
     const double d4 = fmax(0.0, fmax((unt + ct) - (unl + cl), (unr + cr) - (unt + ct)));
     if (l1 < d1) l1 = (l1 * l1 + d1 * d1) / (2 * d1);
     if (l4 < d4) l4 = (l4 * l4 + d4 * d4) / (2 * d4);
-    double d[4];
-    d[0] = l1 * a1 + l2 * a2 + l4 * a4;
-    d[1] = l1 * a1 * (ut - ct * nx) + l2 * a2 * ut + l2 * a3 * (-ny) + l4 * a4 * (ut + ct * nx);
-    d[2] = l1 * a1 * (vt - ct * ny) + l2 * a2 * vt + l2 * a3 * nx + l4 * a4 * (vt + ct * ny);
-    d[3] = l1 * a1 * (ht - unt * ct) + l2 * a2 * 0.5 * q2 + l2 * a3 * utt + l4 * a4 * (ht + unt * ct);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Fh[v] = 0.5 * ((FL[v] + FR[v]) * nx + (GL[v] + GR[v]) * ny - d[v]);
+    const double b1 = l1 * a1, b2 = l2 * a2, b3 = l2 * a3, b4 = l4 * a4;
+    Fh[0] = 0.5 * (Fh[0] - (b1 + b2 + b4));
+    Fh[1] = 0.5 * (Fh[1] - (b1 * (ut - ct * nx) + b2 * ut + b3 * (-ny) + b4 * (ut + ct * nx)));
+    Fh[2] = 0.5 * (Fh[2] - (b1 * (vt - ct * ny) + b2 * vt + b3 * nx + b4 * (vt + ct * ny)));
+    Fh[3] = 0.5 * (Fh[3] - (b1 * (ht - unt * ct) + b2 * 0.5 * q2 + b3 * utt + b4 * (ht + unt * ct)));
 }
 
 struct TileInfo {
@@ -197,10 +200,32 @@ __device__ __forceinline__ double stage_dt(const StageArgs& a, const TileInfo& t
     return dt;
 }
 
-template <int P, int FLUX>
-__device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& ti, const double* sVe) {
+// Neighbour prefetch: the neighbour's modes are loaded one edge ahead (during the volume
+// integral for the first edge), so their latency hides behind the flux arithmetic.  It
+// costs 8*4*Np bytes of registers per lane, affordable up to P2.
+template <int P>
+struct Prefetch {
+    static constexpr bool on = P <= 2;
+};
+
+template <int Np>
+__device__ __forceinline__ void load_modes(const double* __restrict__ U, long long ld, int n, double (&c)[4][Np]) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int i = 0; i < Np; ++i) c[v][i] = U[(v * Np + i) * ld + n];
+}
+
+// STAGED: the tile's own modes and its halo (the out-of-tile neighbours, listed per tile
+// on the host) are first copied into LDS with independent coalesced loads; every
+// neighbour trace is then evaluated from LDS, so no load in the element's arithmetic
+// depends on another load.  Otherwise neighbours are gathered from global memory
+// (prefetched one edge ahead up to P2).
+template <int P, int FLUX, bool STAGED>
+__device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& ti, const double* sVe, double* sU) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqv = D::Nqv, Nqe = D::Nqe;
+    constexpr bool PF = !STAGED && Prefetch<P>::on;
     const DevTables& T = c_tab[P - 1];
     bool done;
     const double dt = stage_dt(a, ti, done);
@@ -224,73 +249,111 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
     const double g = a.gamma;
     double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
 
+    const int* nb_ids = STAGED ? a.nloc : a.nbr;
+    double Uo[4][Np];
     if (e < a.E) {
-        double Uo[4][Np], R[4][Np];
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int i = 0; i < Np; ++i) {
-                Uo[v][i] = a.U[(v * Np + i) * ld + e];
-                R[v][i] = 0.0;
-            }
+            for (int i = 0; i < Np; ++i) Uo[v][i] = a.U[(v * Np + i) * ld + e];
+    }
+    if (STAGED) {
+        const int S = a.lds_stride;
+        if (e < a.E) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int i = 0; i < Np; ++i) sU[(v * Np + i) * S + threadIdx.x] = Uo[v][i];
+        }
+        const int h0 = a.hoff[ti.tile], nh = a.hoff[ti.tile + 1] - h0;
+        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+            const int n = a.hids[h0 + h];
+#pragma unroll
+            for (int r = 0; r < 4 * Np; ++r) sU[r * S + blockDim.x + h] = a.U[r * ld + n];
+        }
+        __syncthreads();
+    }
+
+    if (e < a.E) {
+        const int w0 = nb_ids[e], w1 = nb_ids[ld + e], w2 = nb_ids[2 * ld + e];
+        double R[4][Np], Nb[4][Np];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 0; i < Np; ++i) R[v][i] = 0.0;
         const double area = a.geo[4 * ld + e];
+        const double xix = a.geo[0 * ld + e], xiy = a.geo[1 * ld + e];
+        const double etx = a.geo[2 * ld + e], ety = a.geo[3 * ld + e];
+        double lk_n = a.geo[5 * ld + e], nx_n = a.geo[8 * ld + e], ny_n = a.geo[9 * ld + e];
+        if (PF && w0 >= 0) load_modes<Np>(a.U, ld, w0 >> 2, Nb);
 
         // ---- volume integral: sum_q w_q A (F . grad psi_i) -------------------------
         // The point loop stays rolled: each iteration reads one row of the reference
         // operators into scalar registers instead of keeping every table live.
-        {
-            const double xix = a.geo[0 * ld + e], xiy = a.geo[1 * ld + e];
-            const double etx = a.geo[2 * ld + e], ety = a.geo[3 * ld + e];
 #pragma unroll 1
-            for (int q = 0; q < Nqv; ++q) {
-                double uq[4];
+        for (int q = 0; q < Nqv; ++q) {
+            double uq[4];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    double s = Uo[v][0] * T.Vv[q][0];
+            for (int v = 0; v < 4; ++v) {
+                double s = Uo[v][0] * T.Vv[q][0];
 #pragma unroll
-                    for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Vv[q][i], s);
-                    uq[v] = s;
-                }
-                // contravariant fluxes w A F.(xi_x, xi_y) and w A F.(eta_x, eta_y)
-                const double ir = frcp(uq[0]);
-                const double pr = pressure(g, uq, ir);
-                const double wa = T.wv[q] * area;
-                double Fx[4], Fe[4];
-                euler_dir(g, uq, ir, pr, wa * xix, wa * xiy, Fx);
-                euler_dir(g, uq, ir, pr, wa * etx, wa * ety, Fe);
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-#pragma unroll
-                    for (int i = 1; i < Np; ++i) R[v][i] = fma(Fx[v], T.Dxi[q][i], fma(Fe[v], T.Deta[q][i], R[v][i]));
+                for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Vv[q][i], s);
+                uq[v] = s;
             }
+            // contravariant fluxes w A F.(xi_x, xi_y) and w A F.(eta_x, eta_y)
+            const double ir = frcp(uq[0]);
+            const double pr = pressure(g, uq, ir);
+            const double wa = T.wv[q] * area;
+            double Fx[4], Fe[4];
+            euler_dir(g, uq, ir, pr, wa * xix, wa * xiy, Fx);
+            euler_dir(g, uq, ir, pr, wa * etx, wa * ety, Fe);
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int i = 1; i < Np; ++i) R[v][i] = fma(Fx[v], T.Dxi[q][i], fma(Fe[v], T.Deta[q][i], R[v][i]));
         }
 
         // ---- surface integral, element-owned -------------------------------------
 #pragma unroll 1
         for (int k = 0; k < 3; ++k) {
-            const int w = a.nbr[k * ld + e];
-            const double lk = a.geo[(5 + k) * ld + e];
-            const double nx = a.geo[(8 + 2 * k) * ld + e], ny = a.geo[(9 + 2 * k) * ld + e];
+            const int w = k == 0 ? w0 : (k == 1 ? w1 : w2);
+            const double lk = lk_n, nx = nx_n, ny = ny_n;
             double uR[Nqe][4], mR[4];
             if (w >= 0) {
-                // neighbour trace at the mirrored points, streamed over its modes
-                const int n = w >> 2, sn = w & 3;
-                const double* tb = sVe + sn * (Nqe * Np);
+                // neighbour trace at the mirrored points
+                const double* tb = sVe + (w & 3) * (Nqe * Np);
+                if (STAGED) {
+                    const double* src = sU + (w >> 2);
+                    const int S = a.lds_stride;
 #pragma unroll
-                for (int i = 0; i < Np; ++i) {
-                    double c[4];
+                    for (int i = 0; i < Np; ++i) {
+                        double c[4];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) c[v] = a.U[(v * Np + i) * ld + n];
-                    if (i == 0) {
+                        for (int v = 0; v < 4; ++v) c[v] = src[(v * Np + i) * S];
+                        if (i == 0) {
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) mR[v] = c[v];
+                            for (int v = 0; v < 4; ++v) mR[v] = c[v];
+                        }
+#pragma unroll
+                        for (int j = 0; j < Nqe; ++j) {
+                            const double t = tb[(Nqe - 1 - j) * Np + i];
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) uR[j][v] = i == 0 ? c[v] * t : fma(c[v], t, uR[j][v]);
+                        }
                     }
+                } else {
+                    if (!PF) load_modes<Np>(a.U, ld, w >> 2, Nb);
 #pragma unroll
-                    for (int j = 0; j < Nqe; ++j) {
-                        const double t = tb[(Nqe - 1 - j) * Np + i];
+                    for (int v = 0; v < 4; ++v) mR[v] = Nb[v][0];
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) uR[j][v] = i == 0 ? c[v] * t : fma(c[v], t, uR[j][v]);
-                    }
+                    for (int i = 0; i < Np; ++i)
+#pragma unroll
+                        for (int j = 0; j < Nqe; ++j) {
+                            const double t = tb[(Nqe - 1 - j) * Np + i];
+#pragma unroll
+                            for (int v = 0; v < 4; ++v)
+                                uR[j][v] = i == 0 ? Nb[v][0] * t : fma(Nb[v][i], t, uR[j][v]);
+                        }
                 }
             } else if (w == -RKDG_BC_FARFIELD) {
                 // kernel_boundaryCondition CELL_FARFIELD (src/kernelfunctions.cu:184-201)
@@ -325,6 +388,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
                 mR[3] = Uo[3][0];
                 mR[1] = Uo[1][0] * cxx - cxy * Uo[2][0];
                 mR[2] = Uo[2][0] * cyy - cxy * Uo[1][0];
+            }
+            // next edge's geometry and neighbour modes, in flight during this edge's fluxes
+            if (k < 2) {
+                const int wn = k == 0 ? w1 : w2;
+                lk_n = a.geo[(6 + k) * ld + e];
+                nx_n = a.geo[(10 + 2 * k) * ld + e];
+                ny_n = a.geo[(11 + 2 * k) * ld + e];
+                if (PF && wn >= 0) load_modes<Np>(a.U, ld, wn >> 2, Nb);
             }
 
             double alpha = 0.0;
@@ -414,13 +485,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
 }
 
 // WPE > 0 asks the compiler for at least WPE waves per SIMD (fewer registers, possibly
-// spills); chosen at run time by measurement.
-template <int P, int FLUX, int WPE>
+// spills); chosen at run time by measurement.  The staged form takes its LDS image of the
+// tile and halo as dynamic shared memory: 4 Np rows of lds_stride doubles.
+template <int P, int FLUX, int WPE, bool STAGED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8))) k_stage(StageArgs a) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqe = D::Nqe;
     const DevTables& T = c_tab[P - 1];
     __shared__ double sVe[3 * Nqe * Np];
+    extern __shared__ double sU[];
     for (int t = threadIdx.x; t < 3 * Nqe * Np; t += blockDim.x) {
         const int k = t / (Nqe * Np), r = t - k * (Nqe * Np), j = r / Np, i = r - j * Np;
         sVe[t] = T.Ve[k][j][i];
@@ -430,8 +503,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         TileInfo ti;
         if (!tile_of_iter(a, it, nit, ti)) continue;  // uniform across the workgroup
-        stage_tile<P, FLUX>(a, ti, sVe);
-        __syncthreads();  // the reduction scratch is reused by the next tile
+        stage_tile<P, FLUX, STAGED>(a, ti, sVe, sU);
+        __syncthreads();  // LDS and the reduction scratch are reused by the next tile
     }
 }
 
@@ -602,17 +675,28 @@ __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
 
 // Plain streaming copy (16 bytes per lane per iteration), the calibration kernel for
 // bandwidth and memory counters.
-__global__ void __launch_bounds__(256) k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+__global__ void __launch_bounds__(256) k_stream_copy(const dvec2* __restrict__ src, dvec2* __restrict__ dst,
                                                      long long n) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // four independent 16-byte loads in flight per lane, streamed past the caches
     const long long stride = (long long)gridDim.x * blockDim.x;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        dvec2 v0 = __builtin_nontemporal_load(src + i), v1 = __builtin_nontemporal_load(src + i + stride);
+        dvec2 v2 = __builtin_nontemporal_load(src + i + 2 * stride), v3 = __builtin_nontemporal_load(src + i + 3 * stride);
+        __builtin_nontemporal_store(v0, dst + i);
+        __builtin_nontemporal_store(v1, dst + i + stride);
+        __builtin_nontemporal_store(v2, dst + i + 2 * stride);
+        __builtin_nontemporal_store(v3, dst + i + 3 * stride);
+    }
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
 int launch_stream_copy(const void* src, void* dst, long long bytes, void* stream) {
     const long long n = bytes / 16;
-    hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, (hipStream_t)stream, (const double2*)src,
-                       (double2*)dst, n);
+    hipLaunchKernelGGL(k_stream_copy, dim3(8192), dim3(256), 0, (hipStream_t)stream, (const dvec2*)src,
+                       (dvec2*)dst, n);
     return (int)hipGetLastError();
 }
 
@@ -639,10 +723,20 @@ static int stage_wpe() {
 template <int P, int FLUX>
 static void launch_stage_pf(int block, const StageArgs& a, hipStream_t s) {
     dim3 grid(grid_blocks(a)), blk(block);
+    if (a.staged) {
+        const size_t lds = (size_t)4 * Deg<P>::Np * a.lds_stride * sizeof(double);
+        static size_t lds_set = 65536;  // above 64 KiB the kernel must opt in
+        if (lds > lds_set) {
+            hipFuncSetAttribute((const void*)k_stage<P, FLUX, 0, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            lds_set = lds;
+        }
+        hipLaunchKernelGGL((k_stage<P, FLUX, 0, true>), grid, blk, lds, s, a);
+        return;
+    }
     switch (stage_wpe()) {
-        case 2: hipLaunchKernelGGL((k_stage<P, FLUX, 2>), grid, blk, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_stage<P, FLUX, 3>), grid, blk, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_stage<P, FLUX, 0>), grid, blk, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_stage<P, FLUX, 2, false>), grid, blk, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_stage<P, FLUX, 3, false>), grid, blk, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_stage<P, FLUX, 0, false>), grid, blk, 0, s, a); break;
     }
 }
 
