@@ -210,6 +210,11 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     BCHK(b, hipSetDevice(b->dev));
     BCHK(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     size_t ub = sizeof(double) * hU.size();
+    // the kernels address every array with 32-bit byte offsets
+    if (ub >= (1ull << 32) || sizeof(double) * hgeo.size() >= (1ull << 32)) {
+        rkdg_batch_destroy(b);
+        return bfail(nullptr, RKDG_ERR_ARG, "batch too large (state or geometry array above 4 GiB)");
+    }
     for (int k = 0; k < 3; ++k) {
         BCHK(b, hipMalloc(&b->U[k], ub));
         BCHK(b, hipMemcpy(b->U[k], hU.data(), ub, hipMemcpyHostToDevice));

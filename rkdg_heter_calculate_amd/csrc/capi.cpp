@@ -104,6 +104,8 @@ struct rkdg_ctx {
     int* hids = nullptr;        // staged form: halo element positions
     int lds_stride = 0;         // staged form: doubles per LDS row (block + largest halo)
     int staged = 0;
+    int dma = 0;                // software-pipelined persistent form
+    int persistent = 0;         // workgroups of the persistent grid (0: one per tile)
     unsigned char* dtflag = nullptr;
     double* lim = nullptr;
     int* pairs = nullptr;
@@ -151,6 +153,8 @@ static StageArgs make_args(rkdg_ctx* c, int stage, int slot, int dt_mode, double
     a.hids = c->hids;
     a.lds_stride = c->lds_stride;
     a.staged = c->staged;
+    a.dma = c->dma;
+    a.persistent = c->persistent;
     a.dtflag = c->dtflag;
     a.st = c->st;
     a.ld = c->ld;
@@ -397,9 +401,18 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
         }
     }
 
+    // software-pipelined persistent form for P1/P2 on 256-lane tiles: two workgroups per
+    // CU (its LDS buffer is 44-69 KiB)
+    if (prm->precision == RKDG_PREC_FP64 && c->P <= 2 && c->block == 256 && env_int("RKDG_DMA", 0)) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, prm->device) == hipSuccess && cus > 0) {
+            c->dma = 1;
+            c->persistent = 2 * cus;
+        }
+    }
     // staged form: per-tile halos, used when the tile's LDS image fits
     std::vector<int> hoff, hids, hnloc;
-    if (prm->precision == RKDG_PREC_FP64 && env_int("RKDG_STAGED", 1)) {
+    if (!c->dma && prm->precision == RKDG_PREC_FP64 && env_int("RKDG_STAGED", 1)) {
         const int hmax = build_halo(hnbr.data(), ld, E, c->block, hoff, hids, hnloc);
         const int stride = c->block + hmax;
         const long long lds = (long long)4 * c->Np * stride * (long long)sizeof(double);
@@ -416,6 +429,12 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     size_t ubytes = sizeof(double) * (size_t)4 * c->Np * ld;
+    // the kernels address every array with 32-bit byte offsets
+    if (ubytes >= (1ull << 32) || sizeof(double) * (size_t)GEO_ROWS * ld >= (1ull << 32))
+    {
+        rkdg_destroy(c);
+        return fail(nullptr, RKDG_ERR_ARG, "mesh too large for one context (state or geometry array above 4 GiB)");
+    }
     for (int b = 0; b < 3; ++b) HIPCHK(c, hipMalloc(&c->U[b], ubytes));
     HIPCHK(c, hipMalloc(&c->geo, sizeof(double) * hgeo.size()));
     HIPCHK(c, hipMalloc(&c->nbr, sizeof(int) * hnbr.size()));

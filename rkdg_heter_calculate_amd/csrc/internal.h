@@ -143,6 +143,7 @@ struct StageArgs {
     const int* hids;       // halo element positions, slot blockDim.x + h of tile t is hids[hoff[t] + h]
     int lds_stride;        // >= block + largest halo
     int staged;
+    int dma;               // software-pipelined persistent form (single mesh, P <= 2, 256 lanes)
 };
 
 enum { DTM_DEVICE = 0, DTM_HOST = 1, DTM_FIXED = 2 };

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <climits>
 #include <cstdlib>
+#include <type_traits>
 #include "internal.h"
 
 namespace rkdg {
@@ -33,6 +34,17 @@ struct Deg {
     static constexpr int Nqv = P <= 2 ? 7 : 12;
     static constexpr int Nqe = P <= 2 ? 3 : 4;
 };
+
+// Element e of row r of an SoA array [rows][ld], addressed as the array base (scalar
+// registers) plus one 32-bit byte offset (every array is below 4 GiB, checked when a
+// context is created): the addressing form of global_load/store with an SGPR base, so no
+// 64-bit address is formed per row.
+template <typename T>
+__device__ __forceinline__ T& soa(T* base, int r, long long ld, int e) {
+    const unsigned off = (unsigned)r * (unsigned)(ld * (long long)sizeof(T)) + (unsigned)e * (unsigned)sizeof(T);
+    auto b = reinterpret_cast<typename std::conditional<std::is_const<T>::value, const char*, char*>::type>(base);
+    return *reinterpret_cast<T*>(b + off);
+}
 
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
@@ -213,22 +225,281 @@ __device__ __forceinline__ void load_modes(const double* __restrict__ U, long lo
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-        for (int i = 0; i < Np; ++i) c[v][i] = U[(v * Np + i) * ld + n];
+        for (int i = 0; i < Np; ++i) c[v][i] = soa(U, v * Np + i, ld, n);
 }
 
-// STAGED: the tile's own modes and its halo (the out-of-tile neighbours, listed per tile
-// on the host) are first copied into LDS with independent coalesced loads; every
-// neighbour trace is then evaluated from LDS, so no load in the element's arithmetic
-// depends on another load.  Otherwise neighbours are gathered from global memory
-// (prefetched one edge ahead up to P2).
-template <int P, int FLUX, bool STAGED>
-__device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& ti, const double* sVe, double* sU) {
+// Geometry an element needs before its first edge (the rest is loaded one edge ahead).
+struct ElemGeo {
+    double xix, xiy, etx, ety, area, lk0, nx0, ny0;
+};
+
+// Neighbour sources of the element core.
+enum { NB_GLOBAL = 0, NB_LDS = 1 };
+
+// Residual, SSP-RK3 update and store of one element, from its modes Uo (already in
+// registers), its geometry and its three neighbour codes.  Neighbour traces come from
+// global memory (NB_GLOBAL, prefetched one edge ahead up to P2; the step-start state of
+// stages 1 and 2 is then prefetched into the same registers during the last edge) or
+// from the tile's LDS image (NB_LDS, slots of a.nloc).  Stage 2 folds the element's CFL
+// wave speed and residual into ws_l, r0_l, r1_l (running maxima).
+template <int P, int FLUX, int NBSRC>
+__device__ __forceinline__ void element_core(const StageArgs& a, int e, const double (&Uo)[4][Deg<P>::Np],
+                                             const ElemGeo& eg, int w0, int w1, int w2, double dt, bool dtf,
+                                             const double* sVe, const double* sU, DevState* st, double& ws_l,
+                                             double& r0_l, double& r1_l) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqv = D::Nqv, Nqe = D::Nqe;
-    constexpr bool PF = !STAGED && Prefetch<P>::on;
+    constexpr bool PF = NBSRC == NB_GLOBAL && Prefetch<P>::on;
     const DevTables& T = c_tab[P - 1];
-    bool done;
-    const double dt = stage_dt(a, ti, done);
+    const long long ld = a.ld;
+    const double g = a.gamma;
+    double R[4][Np], Nb[4][Np];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int i = 0; i < Np; ++i) R[v][i] = 0.0;
+    if (PF && w0 >= 0) load_modes<Np>(a.U, ld, w0 >> 2, Nb);
+
+    // ---- volume integral: sum_q w_q A (F . grad psi_i) -----------------------------
+    // The point loop stays rolled: each iteration reads one row of the reference
+    // operators into scalar registers instead of keeping every table live.
+#pragma unroll 1
+    for (int q = 0; q < Nqv; ++q) {
+        double uq[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double s = Uo[v][0] * T.Vv[q][0];
+#pragma unroll
+            for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Vv[q][i], s);
+            uq[v] = s;
+        }
+        // contravariant fluxes w A F.(xi_x, xi_y) and w A F.(eta_x, eta_y)
+        const double ir = frcp(uq[0]);
+        const double pr = pressure(g, uq, ir);
+        const double wa = T.wv[q] * eg.area;
+        double Fx[4], Fe[4];
+        euler_dir(g, uq, ir, pr, wa * eg.xix, wa * eg.xiy, Fx);
+        euler_dir(g, uq, ir, pr, wa * eg.etx, wa * eg.ety, Fe);
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 1; i < Np; ++i) R[v][i] = fma(Fx[v], T.Dxi[q][i], fma(Fe[v], T.Deta[q][i], R[v][i]));
+    }
+
+    // ---- surface integral, element-owned -----------------------------------------
+    // Both loops stay rolled: one quadrature point's traces and flux are live at a time.
+    double lk_n = eg.lk0, nx_n = eg.nx0, ny_n = eg.ny0, per = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {
+        const int w = k == 0 ? w0 : (k == 1 ? w1 : w2);
+        const double lk = lk_n, nx = nx_n, ny = ny_n;
+        per += lk;
+        // neighbour modes: LDS slot, registers (prefetched), or a load now
+        const double* src = nullptr;
+        if (w >= 0) {
+            if (NBSRC == NB_LDS) src = sU + (w >> 2);
+            else if (!PF) load_modes<Np>(a.U, ld, w >> 2, Nb);
+        }
+        const int S = a.lds_stride;
+        const bool refl = w == -RKDG_BC_REFLECTION;
+        const double cxx = refl ? ny * ny - nx * nx : 1.0, cyy = refl ? nx * nx - ny * ny : 1.0;
+        const double cxy = refl ? 2.0 * nx * ny : 0.0;
+        // neighbour (or ghost) cell mean
+        double mR[4];
+        if (w >= 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) mR[v] = NBSRC == NB_LDS ? src[(v * Np) * S] : Nb[v][0];
+        } else if (w == -RKDG_BC_FARFIELD) {
+            // kernel_boundaryCondition CELL_FARFIELD (src/kernelfunctions.cu:184-201)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) mR[v] = a.fs[v];
+        } else {
+            // CELL_REFLECTION (232-268) mirrors the velocity about the edge normal;
+            // CELL_OUTFLOW (203-230) copies the interior trace
+            mR[0] = Uo[0][0];
+            mR[3] = Uo[3][0];
+            mR[1] = Uo[1][0] * cxx - cxy * Uo[2][0];
+            mR[2] = Uo[2][0] * cyy - cxy * Uo[1][0];
+        }
+        double alpha = 0.0;
+        if (FLUX == RKDG_FLUX_LF) {
+            // kernel_calculateLFCoeff (src/kernelfunctions.cu:370-387)
+            const double rho = 0.5 * (Uo[0][0] + mR[0]);
+            const double ir = frcp(rho);
+            const double u = 0.5 * (Uo[1][0] + mR[1]) * ir;
+            const double v = 0.5 * (Uo[2][0] + mR[2]) * ir;
+            const double rE = 0.5 * (Uo[3][0] + mR[3]);
+            const double p = (g - 1.0) * (rE - 0.5 * rho * (u * u + v * v));
+            alpha = fabs(u * nx + v * ny) + sqrt(g * p * ir);
+        }
+        const double hl = 0.5 * lk;
+        const double* tb = sVe + (w & 3) * (Nqe * Np);
+        if constexpr (NBSRC == NB_LDS && FLUX == RKDG_FLUX_LF) {
+            // Registers are not the occupancy limit here (the LDS image is), so all points
+            // of the edge are evaluated together for instruction-level parallelism: the
+            // neighbour trace is streamed from LDS over its modes into Nqe x 4 chains.
+            double uRa[Nqe][4];
+            if (w >= 0) {
+#pragma unroll
+                for (int i = 0; i < Np; ++i) {
+                    double c[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) c[v] = src[(v * Np + i) * S];
+#pragma unroll
+                    for (int j = 0; j < Nqe; ++j) {
+                        const double t = tb[(Nqe - 1 - j) * Np + i];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) uRa[j][v] = i == 0 ? c[v] * t : fma(c[v], t, uRa[j][v]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < Nqe; ++j) {
+                double uL[4], uR[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double s = Uo[v][0] * T.Ve[k][j][0];
+#pragma unroll
+                    for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Ve[k][j][i], s);
+                    uL[v] = s;
+                }
+                if (w >= 0) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) uR[v] = uRa[j][v];
+                } else if (w == -RKDG_BC_FARFIELD) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) uR[v] = a.fs[v];
+                } else {
+                    uR[0] = uL[0];
+                    uR[3] = uL[3];
+                    uR[1] = uL[1] * cxx - cxy * uL[2];
+                    uR[2] = uL[2] * cyy - cxy * uL[1];
+                }
+                double Fh[4];
+                if (FLUX == RKDG_FLUX_LF) {
+                    double FL[4], FR[4];
+                    const double irl = frcp(uL[0]), irr = frcp(uR[0]);
+                    euler_dir(g, uL, irl, pressure(g, uL, irl), nx, ny, FL);
+                    euler_dir(g, uR, irr, pressure(g, uR, irr), nx, ny, FR);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) Fh[v] = 0.5 * (FL[v] + FR[v] - alpha * (uR[v] - uL[v]));
+                } else {
+                    roe_flux(g, uL, uR, nx, ny, Fh);
+                }
+                const double wj = T.we[j] * hl;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const double f = wj * Fh[v];
+#pragma unroll
+                    for (int i = 0; i < Np; ++i) R[v][i] = fma(-f, T.Ve[k][j][i], R[v][i]);
+                }
+            }
+        } else {
+    #pragma unroll 1
+            for (int j = 0; j < Nqe; ++j) {
+                double uL[4], uR[4];
+    #pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double s = Uo[v][0] * T.Ve[k][j][0];
+    #pragma unroll
+                    for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Ve[k][j][i], s);
+                    uL[v] = s;
+                }
+                if (w >= 0) {
+                    // neighbour trace at the mirrored point
+                    const double* t = tb + (Nqe - 1 - j) * Np;
+    #pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        double s = (NBSRC == NB_LDS ? src[(v * Np) * S] : Nb[v][0]) * t[0];
+    #pragma unroll
+                        for (int i = 1; i < Np; ++i) s = fma(NBSRC == NB_LDS ? src[(v * Np + i) * S] : Nb[v][i], t[i], s);
+                        uR[v] = s;
+                    }
+                } else if (w == -RKDG_BC_FARFIELD) {
+    #pragma unroll
+                    for (int v = 0; v < 4; ++v) uR[v] = a.fs[v];
+                } else {
+                    uR[0] = uL[0];
+                    uR[3] = uL[3];
+                    uR[1] = uL[1] * cxx - cxy * uL[2];
+                    uR[2] = uL[2] * cyy - cxy * uL[1];
+                }
+                double Fh[4];
+                if (FLUX == RKDG_FLUX_LF) {
+                    // 0.5 (F(uL).n + F(uR).n - alpha (uR - uL)), kernel_calculateFlux's formula
+                    double FL[4], FR[4];
+                    const double irl = frcp(uL[0]), irr = frcp(uR[0]);
+                    euler_dir(g, uL, irl, pressure(g, uL, irl), nx, ny, FL);
+                    euler_dir(g, uR, irr, pressure(g, uR, irr), nx, ny, FR);
+    #pragma unroll
+                    for (int v = 0; v < 4; ++v) Fh[v] = 0.5 * (FL[v] + FR[v] - alpha * (uR[v] - uL[v]));
+                } else {
+                    roe_flux(g, uL, uR, nx, ny, Fh);
+                }
+                const double wj = T.we[j] * hl;
+    #pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const double f = wj * Fh[v];
+    #pragma unroll
+                    for (int i = 0; i < Np; ++i) R[v][i] = fma(-f, T.Ve[k][j][i], R[v][i]);
+                }
+            }
+        }
+        // next edge's geometry and neighbour modes (after the last edge: the step-start
+        // state), issued once this edge no longer needs the registers
+        if (k < 2) {
+            const int wn = k == 0 ? w1 : w2;
+            lk_n = soa(a.geo, 6 + k, ld, e);
+            nx_n = soa(a.geo, 10 + 2 * k, ld, e);
+            ny_n = soa(a.geo, 11 + 2 * k, ld, e);
+            if (PF && wn >= 0) load_modes<Np>(a.U, ld, wn >> 2, Nb);
+        } else if (PF && a.stage > 0) {
+            load_modes<Np>(a.U0, ld, e, Nb);
+        }
+    }
+
+    // ---- SSP-RK3 stage (src/kernelfunctions.cu:640-748) ---------------------------
+    const double ia = frcp(eg.area);
+    double mean[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int i = 0; i < Np; ++i) {
+            const double r = R[v][i] * (T.minv[i] * ia);
+            double out;
+            if (a.stage == 0) {
+                out = Uo[v][i] + dt * r;
+            } else {
+                const double o0 = PF ? Nb[v][i] : soa(a.U0, v * Np + i, ld, e);
+                if (a.stage == 1) {
+                    out = 0.75 * o0 + 0.25 * Uo[v][i] + 0.25 * dt * r;
+                } else {
+                    out = (1.0 / 3.0) * o0 + (2.0 / 3.0) * Uo[v][i] + (2.0 / 3.0) * dt * r;
+                    if (i == 0 && dtf) {
+                        if (v == 0) r0_l = fmax(r0_l, fabs(out - o0));
+                        if (v == 3) r1_l = fmax(r1_l, fabs(out - o0));
+                    }
+                }
+            }
+            if (i == 0) mean[v] = out;
+            soa(a.Uout, v * Np + i, ld, e) = out;
+        }
+
+    if (a.stage == 2) {
+        // next step's CFL wave speed on the new means (kernel_getTimeStep's formula,
+        // src/kernelfunctions.cu:19-25)
+        const double rho = mean[0], u = mean[1] / rho, v = mean[2] / rho;
+        const double p = (g - 1.0) * (mean[3] - 0.5 * rho * (u * u + v * v));
+        if (!(rho > 0.0) || !(p > 0.0)) atomicMin(&st->err_pos, e);
+        const double c = sqrt(g * p / rho);
+        const double s = (sqrt(u * u + v * v) + c) * per / eg.area;
+        if (dtf && s > 0.0) ws_l = fmax(ws_l, s);
+    }
+}
+
+// Per-mesh bookkeeping done once per stage by the workgroup owning the mesh's first tile:
+// stage 0 fixes the step's dt and clears the reductions, stage 2 advances the time.
+__device__ __forceinline__ void stage_bookkeeping(const StageArgs& a, const TileInfo& ti, double dt, bool done) {
     if (ti.first && threadIdx.x == 0) {
         if (a.stage == 0 && !done) {
             ti.st->ws[1 - a.slot] = 0ull;
@@ -242,11 +513,21 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
             else ti.st->nsteps += 1;
         }
     }
+}
+
+// One workgroup tile, operands loaded from global memory.  STAGED: the tile's own modes
+// and its halo (the out-of-tile neighbours, listed per tile on the host) are first copied
+// into LDS, and every neighbour trace is evaluated from LDS.
+template <int P, int FLUX, bool STAGED>
+__device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& ti, const double* sVe, double* sU) {
+    constexpr int Np = Deg<P>::Np;
+    bool done;
+    const double dt = stage_dt(a, ti, done);
+    stage_bookkeeping(a, ti, dt, done);
     if (done) return;
 
     const long long ld = a.ld;
     const int e = ti.tile * blockDim.x + threadIdx.x;
-    const double g = a.gamma;
     double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
 
     const int* nb_ids = STAGED ? a.nloc : a.nbr;
@@ -255,7 +536,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int i = 0; i < Np; ++i) Uo[v][i] = a.U[(v * Np + i) * ld + e];
+            for (int i = 0; i < Np; ++i) Uo[v][i] = soa(a.U, v * Np + i, ld, e);
     }
     if (STAGED) {
         const int S = a.lds_stride;
@@ -269,226 +550,34 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
         for (int h = threadIdx.x; h < nh; h += blockDim.x) {
             const int n = a.hids[h0 + h];
 #pragma unroll
-            for (int r = 0; r < 4 * Np; ++r) sU[r * S + blockDim.x + h] = a.U[r * ld + n];
+            for (int r = 0; r < 4 * Np; ++r) sU[r * S + blockDim.x + h] = soa(a.U, r, ld, n);
         }
         __syncthreads();
     }
 
     if (e < a.E) {
-        const int w0 = nb_ids[e], w1 = nb_ids[ld + e], w2 = nb_ids[2 * ld + e];
-        double R[4][Np], Nb[4][Np];
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int i = 0; i < Np; ++i) R[v][i] = 0.0;
-        const double area = a.geo[4 * ld + e];
-        const double xix = a.geo[0 * ld + e], xiy = a.geo[1 * ld + e];
-        const double etx = a.geo[2 * ld + e], ety = a.geo[3 * ld + e];
-        double lk_n = a.geo[5 * ld + e], nx_n = a.geo[8 * ld + e], ny_n = a.geo[9 * ld + e];
-        if (PF && w0 >= 0) load_modes<Np>(a.U, ld, w0 >> 2, Nb);
-
-        // ---- volume integral: sum_q w_q A (F . grad psi_i) -------------------------
-        // The point loop stays rolled: each iteration reads one row of the reference
-        // operators into scalar registers instead of keeping every table live.
-#pragma unroll 1
-        for (int q = 0; q < Nqv; ++q) {
-            double uq[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                double s = Uo[v][0] * T.Vv[q][0];
-#pragma unroll
-                for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Vv[q][i], s);
-                uq[v] = s;
-            }
-            // contravariant fluxes w A F.(xi_x, xi_y) and w A F.(eta_x, eta_y)
-            const double ir = frcp(uq[0]);
-            const double pr = pressure(g, uq, ir);
-            const double wa = T.wv[q] * area;
-            double Fx[4], Fe[4];
-            euler_dir(g, uq, ir, pr, wa * xix, wa * xiy, Fx);
-            euler_dir(g, uq, ir, pr, wa * etx, wa * ety, Fe);
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-#pragma unroll
-                for (int i = 1; i < Np; ++i) R[v][i] = fma(Fx[v], T.Dxi[q][i], fma(Fe[v], T.Deta[q][i], R[v][i]));
-        }
-
-        // ---- surface integral, element-owned -------------------------------------
-#pragma unroll 1
-        for (int k = 0; k < 3; ++k) {
-            const int w = k == 0 ? w0 : (k == 1 ? w1 : w2);
-            const double lk = lk_n, nx = nx_n, ny = ny_n;
-            double uR[Nqe][4], mR[4];
-            if (w >= 0) {
-                // neighbour trace at the mirrored points
-                const double* tb = sVe + (w & 3) * (Nqe * Np);
-                if (STAGED) {
-                    const double* src = sU + (w >> 2);
-                    const int S = a.lds_stride;
-#pragma unroll
-                    for (int i = 0; i < Np; ++i) {
-                        double c[4];
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) c[v] = src[(v * Np + i) * S];
-                        if (i == 0) {
-#pragma unroll
-                            for (int v = 0; v < 4; ++v) mR[v] = c[v];
-                        }
-#pragma unroll
-                        for (int j = 0; j < Nqe; ++j) {
-                            const double t = tb[(Nqe - 1 - j) * Np + i];
-#pragma unroll
-                            for (int v = 0; v < 4; ++v) uR[j][v] = i == 0 ? c[v] * t : fma(c[v], t, uR[j][v]);
-                        }
-                    }
-                } else {
-                    if (!PF) load_modes<Np>(a.U, ld, w >> 2, Nb);
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) mR[v] = Nb[v][0];
-#pragma unroll
-                    for (int i = 0; i < Np; ++i)
-#pragma unroll
-                        for (int j = 0; j < Nqe; ++j) {
-                            const double t = tb[(Nqe - 1 - j) * Np + i];
-#pragma unroll
-                            for (int v = 0; v < 4; ++v)
-                                uR[j][v] = i == 0 ? Nb[v][0] * t : fma(Nb[v][i], t, uR[j][v]);
-                        }
-                }
-            } else if (w == -RKDG_BC_FARFIELD) {
-                // kernel_boundaryCondition CELL_FARFIELD (src/kernelfunctions.cu:184-201)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    mR[v] = a.fs[v];
-#pragma unroll
-                    for (int j = 0; j < Nqe; ++j) uR[j][v] = a.fs[v];
-                }
-            } else {
-                // CELL_REFLECTION (232-268) mirrors the velocity about the edge normal;
-                // CELL_OUTFLOW (203-230) copies the interior trace
-                const bool refl = w == -RKDG_BC_REFLECTION;
-                const double cxx = refl ? ny * ny - nx * nx : 1.0, cyy = refl ? nx * nx - ny * ny : 1.0;
-                const double cxy = refl ? 2.0 * nx * ny : 0.0;
-#pragma unroll
-                for (int j = 0; j < Nqe; ++j) {
-                    double uL[4];
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        double s = Uo[v][0] * T.Ve[k][j][0];
-#pragma unroll
-                        for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Ve[k][j][i], s);
-                        uL[v] = s;
-                    }
-                    uR[j][0] = uL[0];
-                    uR[j][3] = uL[3];
-                    uR[j][1] = uL[1] * cxx - cxy * uL[2];
-                    uR[j][2] = uL[2] * cyy - cxy * uL[1];
-                }
-                mR[0] = Uo[0][0];
-                mR[3] = Uo[3][0];
-                mR[1] = Uo[1][0] * cxx - cxy * Uo[2][0];
-                mR[2] = Uo[2][0] * cyy - cxy * Uo[1][0];
-            }
-            // next edge's geometry and neighbour modes, in flight during this edge's fluxes
-            if (k < 2) {
-                const int wn = k == 0 ? w1 : w2;
-                lk_n = a.geo[(6 + k) * ld + e];
-                nx_n = a.geo[(10 + 2 * k) * ld + e];
-                ny_n = a.geo[(11 + 2 * k) * ld + e];
-                if (PF && wn >= 0) load_modes<Np>(a.U, ld, wn >> 2, Nb);
-            }
-
-            double alpha = 0.0;
-            if (FLUX == RKDG_FLUX_LF) {
-                // kernel_calculateLFCoeff (src/kernelfunctions.cu:370-387)
-                const double rho = 0.5 * (Uo[0][0] + mR[0]);
-                const double ir = frcp(rho);
-                const double u = 0.5 * (Uo[1][0] + mR[1]) * ir;
-                const double v = 0.5 * (Uo[2][0] + mR[2]) * ir;
-                const double rE = 0.5 * (Uo[3][0] + mR[3]);
-                const double p = (g - 1.0) * (rE - 0.5 * rho * (u * u + v * v));
-                alpha = fabs(u * nx + v * ny) + sqrt(g * p * ir);
-            }
-            const double hl = 0.5 * lk;
-#pragma unroll
-            for (int j = 0; j < Nqe; ++j) {
-                double uL[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    double s = Uo[v][0] * T.Ve[k][j][0];
-#pragma unroll
-                    for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Ve[k][j][i], s);
-                    uL[v] = s;
-                }
-                double Fh[4];
-                if (FLUX == RKDG_FLUX_LF) {
-                    // 0.5 (F(uL).n + F(uR).n - alpha (uR - uL)), kernel_calculateFlux's formula
-                    double FL[4], FR[4];
-                    const double irl = frcp(uL[0]), irr = frcp(uR[j][0]);
-                    euler_dir(g, uL, irl, pressure(g, uL, irl), nx, ny, FL);
-                    euler_dir(g, uR[j], irr, pressure(g, uR[j], irr), nx, ny, FR);
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) Fh[v] = 0.5 * (FL[v] + FR[v] - alpha * (uR[j][v] - uL[v]));
-                } else {
-                    roe_flux(g, uL, uR[j], nx, ny, Fh);
-                }
-                const double wj = T.we[j] * hl;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const double f = wj * Fh[v];
-#pragma unroll
-                    for (int i = 0; i < Np; ++i) R[v][i] = fma(-f, T.Ve[k][j][i], R[v][i]);
-                }
-            }
-        }
-
-        // ---- SSP-RK3 stage (src/kernelfunctions.cu:640-748) -----------------------
-        const double ia = frcp(area);
-        double mean[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int i = 0; i < Np; ++i) {
-                const long long idx = (v * Np + i) * ld + e;
-                const double r = R[v][i] * (T.minv[i] * ia);
-                double out;
-                if (a.stage == 0) {
-                    out = Uo[v][i] + dt * r;
-                } else if (a.stage == 1) {
-                    out = 0.75 * a.U0[idx] + 0.25 * Uo[v][i] + 0.25 * dt * r;
-                } else {
-                    const double o0 = a.U0[idx];
-                    out = (1.0 / 3.0) * o0 + (2.0 / 3.0) * Uo[v][i] + (2.0 / 3.0) * dt * r;
-                    if (i == 0 && a.dtflag[e]) {
-                        if (v == 0) r0_l = fabs(out - o0);
-                        if (v == 3) r1_l = fabs(out - o0);
-                    }
-                }
-                if (i == 0) mean[v] = out;
-                a.Uout[idx] = out;
-            }
-
-        if (a.stage == 2) {
-            // next step's CFL wave speed on the new means (kernel_getTimeStep's formula,
-            // src/kernelfunctions.cu:19-25)
-            const double rho = mean[0], u = mean[1] / rho, v = mean[2] / rho;
-            const double p = (g - 1.0) * (mean[3] - 0.5 * rho * (u * u + v * v));
-            if (!(rho > 0.0) || !(p > 0.0)) atomicMin(&ti.st->err_pos, e);
-            const double c = sqrt(g * p / rho);
-            const double per = a.geo[5 * ld + e] + a.geo[6 * ld + e] + a.geo[7 * ld + e];
-            const double s = (sqrt(u * u + v * v) + c) * per / area;
-            if (a.dtflag[e] && s > 0.0) ws_l = s;
-        }
+        const int w0 = soa(nb_ids, 0, ld, e), w1 = soa(nb_ids, 1, ld, e), w2 = soa(nb_ids, 2, ld, e);
+        ElemGeo eg;
+        eg.area = soa(a.geo, 4, ld, e);
+        eg.xix = soa(a.geo, 0, ld, e);
+        eg.xiy = soa(a.geo, 1, ld, e);
+        eg.etx = soa(a.geo, 2, ld, e);
+        eg.ety = soa(a.geo, 3, ld, e);
+        eg.lk0 = soa(a.geo, 5, ld, e);
+        eg.nx0 = soa(a.geo, 8, ld, e);
+        eg.ny0 = soa(a.geo, 9, ld, e);
+        const bool dtf = a.dtflag[e] != 0;
+        element_core<P, FLUX, STAGED ? NB_LDS : NB_GLOBAL>(a, e, Uo, eg, w0, w1, w2, dt, dtf, sVe, sU, ti.st, ws_l,
+                                                          r0_l, r1_l);
     }
     if (a.stage == 2)
         block_max3_atomic(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], &ti.st->res[0], &ti.st->res[1], true);
 }
 
-// WPE > 0 asks the compiler for at least WPE waves per SIMD (fewer registers, possibly
-// spills); chosen at run time by measurement.  The staged form takes its LDS image of the
-// tile and halo as dynamic shared memory: 4 Np rows of lds_stride doubles.
-template <int P, int FLUX, int WPE, bool STAGED>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8))) k_stage(StageArgs a) {
+// One workgroup per tile, operands from global memory (STAGED: neighbours from the
+// tile's LDS image, passed as dynamic shared memory of 4 Np rows of lds_stride doubles).
+template <int P, int FLUX, bool STAGED>
+__global__ void __launch_bounds__(256) k_stage(StageArgs a) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqe = D::Nqe;
     const DevTables& T = c_tab[P - 1];
@@ -506,6 +595,116 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 
         stage_tile<P, FLUX, STAGED>(a, ti, sVe, sU);
         __syncthreads();  // LDS and the reduction scratch are reused by the next tile
     }
+}
+
+// ---- software-pipelined persistent form -----------------------------------------------
+// Each workgroup (256 lanes, tiles of 256 elements) walks its tiles in order.  While it
+// computes tile t from registers, the LDS direct-load engine (global_load_lds_dwordx4,
+// no registers involved) brings tile t+1's modes, the geometry used before the first edge
+// and the neighbour codes into one LDS buffer; neighbours are gathered from global memory
+// one edge ahead and the step-start state during the last edge.  Memory traffic thus
+// overlaps the arithmetic inside every wave instead of relying on other waves.
+template <int P>
+struct DmaLayout {
+    static constexpr int Np = Deg<P>::Np;
+    static constexpr int NR = 4 * Np;      // modal rows
+    static constexpr int NG = 8;           // geometry rows 0-5, 8, 9
+    static constexpr int TILE = 256;
+    static constexpr int CHUNKS = 2 * (NR + NG) + 3;   // 1 KiB per wave-wide 16-byte load
+    static constexpr size_t BYTES = (size_t)(NR + NG) * TILE * 8 + 3 * TILE * 4;
+};
+
+__device__ __forceinline__ int dma_geo_row(int gi) { return gi < 6 ? gi : gi + 2; }
+
+// Issues this wave's share of the LDS direct loads of tile t.
+template <int P>
+__device__ __forceinline__ void dma_issue(const StageArgs& a, int t, char* buf) {
+    using L = DmaLayout<P>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const long long ld = a.ld;
+    const long long e0 = (long long)t * L::TILE;
+    for (int c = wave; c < L::CHUNKS; c += nwave) {
+        const char* src;
+        int dst;
+        if (c < 2 * L::NR) {
+            const int r = c >> 1, h = c & 1;
+            src = (const char*)(a.U + r * ld + e0) + h * 1024;
+            dst = r * 2048 + h * 1024;
+        } else if (c < 2 * (L::NR + L::NG)) {
+            const int gi = (c - 2 * L::NR) >> 1, h = c & 1;
+            src = (const char*)(a.geo + dma_geo_row(gi) * ld + e0) + h * 1024;
+            dst = (L::NR + gi) * 2048 + h * 1024;
+        } else {
+            const int k = c - 2 * (L::NR + L::NG);
+            src = (const char*)(a.nbr + k * ld + e0);
+            dst = (L::NR + L::NG) * 2048 + k * 1024;
+        }
+        __builtin_amdgcn_global_load_lds((const void*)(src + lane * 16),
+                                         (__attribute__((address_space(3))) void*)(buf + dst), 16, 0, 0);
+    }
+}
+
+template <int P, int FLUX>
+__global__ void __launch_bounds__(256) k_stage_dma(StageArgs a) {
+    using D = Deg<P>;
+    using L = DmaLayout<P>;
+    constexpr int Np = D::Np, Nqe = D::Nqe;
+    const DevTables& T = c_tab[P - 1];
+    __shared__ double sVe[3 * Nqe * Np];
+    extern __shared__ double dbuf[];
+    char* buf = (char*)dbuf;
+    for (int t = threadIdx.x; t < 3 * Nqe * Np; t += blockDim.x) {
+        const int k = t / (Nqe * Np), r = t - k * (Nqe * Np), j = r / Np, i = r - j * Np;
+        sVe[t] = T.Ve[k][j][i];
+    }
+    const int nit = iter_count(a);
+    const int G = gridDim.x;
+    const int tid = threadIdx.x;
+    double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
+    DevState* st = a.st;
+
+    int it = blockIdx.x;
+    TileInfo ti;
+    bool have = it < nit && tile_of_iter(a, it, nit, ti);
+    if (have) dma_issue<P>(a, ti.tile, buf);
+    while (have) {
+        bool done;
+        const double dt = stage_dt(a, ti, done);
+        stage_bookkeeping(a, ti, dt, done);
+        // this tile's LDS image is complete once every wave's direct loads have landed
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        const int e = ti.tile * L::TILE + tid;
+        double Uo[4][Np];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 0; i < Np; ++i) Uo[v][i] = dbuf[(v * Np + i) * L::TILE + tid];
+        ElemGeo eg;
+        const double* gb = dbuf + L::NR * L::TILE;
+        eg.xix = gb[0 * L::TILE + tid];
+        eg.xiy = gb[1 * L::TILE + tid];
+        eg.etx = gb[2 * L::TILE + tid];
+        eg.ety = gb[3 * L::TILE + tid];
+        eg.area = gb[4 * L::TILE + tid];
+        eg.lk0 = gb[5 * L::TILE + tid];
+        eg.nx0 = gb[6 * L::TILE + tid];
+        eg.ny0 = gb[7 * L::TILE + tid];
+        const int* nb = (const int*)(buf + (size_t)(L::NR + L::NG) * L::TILE * 8);
+        const int w0 = nb[tid], w1 = nb[L::TILE + tid], w2 = nb[2 * L::TILE + tid];
+        __syncthreads();  // the buffer is free: start the next tile's loads
+        it += G;
+        TileInfo tn;
+        const bool have_n = it < nit && tile_of_iter(a, it, nit, tn);
+        if (have_n) dma_issue<P>(a, tn.tile, buf);
+        if (!done && e < a.E) {
+            const bool dtf = a.stage == 2 && a.dtflag[e] != 0;
+            element_core<P, FLUX, NB_GLOBAL>(a, e, Uo, eg, w0, w1, w2, dt, dtf, sVe, nullptr, st, ws_l, r0_l, r1_l);
+        }
+        ti = tn;
+        have = have_n;
+    }
+    if (a.stage == 2) block_max3_atomic(ws_l, r0_l, r1_l, &st->ws[1 - a.slot], &st->res[0], &st->res[1], true);
 }
 
 #include "stage_mx.inc"
@@ -712,32 +911,31 @@ static int grid_blocks(const StageArgs& a) {
     return nb;
 }
 
-static int stage_wpe() {
-    static int w = [] {
-        const char* e = std::getenv("RKDG_WPE");
-        return e ? std::atoi(e) : 0;
-    }();
-    return w;
-}
-
 template <int P, int FLUX>
 static void launch_stage_pf(int block, const StageArgs& a, hipStream_t s) {
     dim3 grid(grid_blocks(a)), blk(block);
+    if (a.dma && P <= 2) {
+        constexpr size_t lds = DmaLayout<P <= 2 ? P : 2>::BYTES;
+        static bool opted = false;  // above 64 KiB the kernel must opt in
+        if (!opted) {
+            hipFuncSetAttribute((const void*)k_stage_dma<P <= 2 ? P : 2, FLUX>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            opted = true;
+        }
+        hipLaunchKernelGGL((k_stage_dma<P <= 2 ? P : 2, FLUX>), grid, blk, lds, s, a);
+        return;
+    }
     if (a.staged) {
         const size_t lds = (size_t)4 * Deg<P>::Np * a.lds_stride * sizeof(double);
         static size_t lds_set = 65536;  // above 64 KiB the kernel must opt in
         if (lds > lds_set) {
-            hipFuncSetAttribute((const void*)k_stage<P, FLUX, 0, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipFuncSetAttribute((const void*)k_stage<P, FLUX, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             lds_set = lds;
         }
-        hipLaunchKernelGGL((k_stage<P, FLUX, 0, true>), grid, blk, lds, s, a);
+        hipLaunchKernelGGL((k_stage<P, FLUX, true>), grid, blk, lds, s, a);
         return;
     }
-    switch (stage_wpe()) {
-        case 2: hipLaunchKernelGGL((k_stage<P, FLUX, 2, false>), grid, blk, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_stage<P, FLUX, 3, false>), grid, blk, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_stage<P, FLUX, 0, false>), grid, blk, 0, s, a); break;
-    }
+    hipLaunchKernelGGL((k_stage<P, FLUX, false>), grid, blk, 0, s, a);
 }
 
 template <int P>

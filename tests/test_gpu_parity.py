@@ -157,7 +157,10 @@ def test_batch_equals_single():
         s.advance(9)
         Ub = np.empty((4, 6, gr.n_elements))
         L.check(lib.rkdg_batch_get_state(h, i, 1, L.dptr(Ub)), None, "get")
-        assert np.array_equal(Ub, s.get_state(basis=1)), i
+        # both are FP64 forms of the same scheme; they may fuse multiply-adds differently
+        Us = s.get_state(basis=1)
+        for v in range(4):
+            assert np.linalg.norm(Ub[v] - Us[v]) <= 1e-13 * np.linalg.norm(Us[v]), (i, v)
     lib.rkdg_batch_destroy(h)
 
 
