@@ -208,7 +208,7 @@ def main():
     B = algorithmic_bytes_per_element_stage(Np)
     per_launch = B * E
     achieved = per_launch / (r["stage_ms"] / 1e3) / 1e9
-    traffic = read_traffic(f"stage_p{P}_{args.n}x{args.n}_pmc.txt")
+    traffic = read_traffic(f"k_stage_p{P}_lf_{args.n}x{args.n}_hbm.md")
 
     cpu = None
     if not args.no_cpu:

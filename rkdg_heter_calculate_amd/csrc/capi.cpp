@@ -412,15 +412,12 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     }
     // staged form: per-tile halos, used when the tile's LDS image fits
     std::vector<int> hoff, hids, hnloc;
-    if (!c->dma && prm->precision == RKDG_PREC_FP64 && env_int("RKDG_STAGED", 1)) {
+    if (!c->dma && prm->precision == RKDG_PREC_FP64 && c->block == staged_block(c->P) && env_int("RKDG_STAGED", 1)) {
         const int hmax = build_halo(hnbr.data(), ld, E, c->block, hoff, hids, hnloc);
-        const int stride = c->block + hmax;
-        const long long lds = (long long)4 * c->Np * stride * (long long)sizeof(double);
-        const long long cap = std::min<long long>(max_lds_per_block(prm->device),
-                                                  (long long)env_int("RKDG_STAGED_MAX_KB", 80) * 1024);
-        if (lds <= cap) {
+        const long long lds = (long long)4 * c->Np * staged_stride(c->P) * (long long)sizeof(double);
+        if (c->block + hmax <= staged_stride(c->P) && lds <= max_lds_per_block(prm->device)) {
             c->staged = 1;
-            c->lds_stride = stride;
+            c->lds_stride = staged_stride(c->P);
         }
     }
 

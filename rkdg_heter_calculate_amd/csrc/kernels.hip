@@ -294,13 +294,19 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
         const int w = k == 0 ? w0 : (k == 1 ? w1 : w2);
         const double lk = lk_n, nx = nx_n, ny = ny_n;
         per += lk;
+        if (NBSRC == NB_LDS && k < 2) {
+            // next edge's geometry, in flight during this edge
+            lk_n = soa(a.geo, 6 + k, ld, e);
+            nx_n = soa(a.geo, 10 + 2 * k, ld, e);
+            ny_n = soa(a.geo, 11 + 2 * k, ld, e);
+        }
         // neighbour modes: LDS slot, registers (prefetched), or a load now
         const double* src = nullptr;
         if (w >= 0) {
             if (NBSRC == NB_LDS) src = sU + (w >> 2);
             else if (!PF) load_modes<Np>(a.U, ld, w >> 2, Nb);
         }
-        const int S = a.lds_stride;
+        constexpr int S = staged_stride(P);
         const bool refl = w == -RKDG_BC_REFLECTION;
         const double cxx = refl ? ny * ny - nx * nx : 1.0, cyy = refl ? nx * nx - ny * ny : 1.0;
         const double cxy = refl ? 2.0 * nx * ny : 0.0;
@@ -334,7 +340,7 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
         }
         const double hl = 0.5 * lk;
         const double* tb = sVe + (w & 3) * (Nqe * Np);
-        if constexpr (NBSRC == NB_LDS && FLUX == RKDG_FLUX_LF) {
+        if constexpr (NBSRC == NB_LDS && FLUX == RKDG_FLUX_LF && P <= 2) {
             // Registers are not the occupancy limit here (the LDS image is), so all points
             // of the edge are evaluated together for instruction-level parallelism: the
             // neighbour trace is streamed from LDS over its modes into Nqe x 4 chains.
@@ -449,9 +455,11 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
         // state), issued once this edge no longer needs the registers
         if (k < 2) {
             const int wn = k == 0 ? w1 : w2;
-            lk_n = soa(a.geo, 6 + k, ld, e);
-            nx_n = soa(a.geo, 10 + 2 * k, ld, e);
-            ny_n = soa(a.geo, 11 + 2 * k, ld, e);
+            if (NBSRC != NB_LDS) {
+                lk_n = soa(a.geo, 6 + k, ld, e);
+                nx_n = soa(a.geo, 10 + 2 * k, ld, e);
+                ny_n = soa(a.geo, 11 + 2 * k, ld, e);
+            }
             if (PF && wn >= 0) load_modes<Np>(a.U, ld, wn >> 2, Nb);
         } else if (PF && a.stage > 0) {
             load_modes<Np>(a.U0, ld, e, Nb);
@@ -532,32 +540,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
 
     const int* nb_ids = STAGED ? a.nloc : a.nbr;
     double Uo[4][Np];
-    if (e < a.E) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int i = 0; i < Np; ++i) Uo[v][i] = soa(a.U, v * Np + i, ld, e);
-    }
-    if (STAGED) {
-        const int S = a.lds_stride;
-        if (e < a.E) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-#pragma unroll
-                for (int i = 0; i < Np; ++i) sU[(v * Np + i) * S + threadIdx.x] = Uo[v][i];
-        }
-        const int h0 = a.hoff[ti.tile], nh = a.hoff[ti.tile + 1] - h0;
-        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
-            const int n = a.hids[h0 + h];
-#pragma unroll
-            for (int r = 0; r < 4 * Np; ++r) sU[r * S + blockDim.x + h] = soa(a.U, r, ld, n);
-        }
-        __syncthreads();
-    }
-
-    if (e < a.E) {
-        const int w0 = soa(nb_ids, 0, ld, e), w1 = soa(nb_ids, 1, ld, e), w2 = soa(nb_ids, 2, ld, e);
-        ElemGeo eg;
+    int w0 = 0, w1 = 0, w2 = 0;
+    ElemGeo eg{};
+    bool dtf = false;
+    auto load_element = [&]() {
+        w0 = soa(nb_ids, 0, ld, e);
+        w1 = soa(nb_ids, 1, ld, e);
+        w2 = soa(nb_ids, 2, ld, e);
         eg.area = soa(a.geo, 4, ld, e);
         eg.xix = soa(a.geo, 0, ld, e);
         eg.xiy = soa(a.geo, 1, ld, e);
@@ -566,23 +555,76 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
         eg.lk0 = soa(a.geo, 5, ld, e);
         eg.nx0 = soa(a.geo, 8, ld, e);
         eg.ny0 = soa(a.geo, 9, ld, e);
-        const bool dtf = a.dtflag[e] != 0;
+        dtf = a.stage == 2 && a.dtflag[e] != 0;
+    };
+    if (STAGED) {
+        // The LDS image is filled by the direct-to-LDS load engine in one batch: the own
+        // rows as 16-byte lanes (1 KiB per wave instruction), the halo as 4-byte lanes (two
+        // lanes per double, 32 halo elements per instruction).  No data passes through
+        // registers, and the element's own operands are loaded in the same round trip.
+        constexpr int S = staged_stride(P), B = staged_block(P), NR = 4 * Np;
+        constexpr int NW = B / 64, PARTS = B / 128, HCH = (S - B + 31) / 32;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        char* lds = (char*)sU;
+        const int h0 = a.hoff[ti.tile], nh = a.hoff[ti.tile + 1] - h0;
+        int hid[HCH];
+#pragma unroll
+        for (int hb = 0; hb < HCH; ++hb) {
+            const int h = hb * 32 + (lane >> 1);
+            hid[hb] = h < nh ? a.hids[h0 + h] : 0;
+        }
+        const long long e0 = (long long)ti.tile * B;
+        for (int c = wave; c < NR * PARTS; c += NW) {
+            const int r = c / PARTS, part = c - r * PARTS;
+            const char* src = (const char*)(a.U + r * ld + e0) + part * 1024 + lane * 16;
+            __builtin_amdgcn_global_load_lds((const void*)src,
+                                             (__attribute__((address_space(3))) void*)(lds + r * S * 8 + part * 1024), 16, 0, 0);
+        }
+        if (e < a.E) load_element();
+        for (int c = wave; c < NR * HCH; c += NW) {
+            const int r = c / HCH, hb = c - r * HCH;
+            if (hb * 32 >= nh) continue;  // uniform across the wave
+            int id = hid[0];
+#pragma unroll
+            for (int q = 1; q < HCH; ++q) id = hb == q ? hid[q] : id;
+            if (hb * 32 + (lane >> 1) < nh) {
+                const char* src = (const char*)(a.U + r * ld) + (size_t)id * 8 + (lane & 1) * 4;
+                __builtin_amdgcn_global_load_lds((const void*)src,
+                                                 (__attribute__((address_space(3))) void*)(lds + (r * S + B + hb * 32) * 8),
+                                                 4, 0, 0);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 0; i < Np; ++i) Uo[v][i] = sU[(v * Np + i) * S + threadIdx.x];
+    } else if (e < a.E) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 0; i < Np; ++i) Uo[v][i] = soa(a.U, v * Np + i, ld, e);
+        load_element();
+    }
+
+    if (e < a.E)
         element_core<P, FLUX, STAGED ? NB_LDS : NB_GLOBAL>(a, e, Uo, eg, w0, w1, w2, dt, dtf, sVe, sU, ti.st, ws_l,
                                                           r0_l, r1_l);
-    }
     if (a.stage == 2)
         block_max3_atomic(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], &ti.st->res[0], &ti.st->res[1], true);
 }
 
 // One workgroup per tile, operands from global memory (STAGED: neighbours from the
-// tile's LDS image, passed as dynamic shared memory of 4 Np rows of lds_stride doubles).
+// tile's LDS image, passed as dynamic shared memory of 4 Np rows of staged_stride(P)
+// doubles; the workgroup then has staged_block(P) lanes).
 template <int P, int FLUX, bool STAGED>
 __global__ void __launch_bounds__(256) k_stage(StageArgs a) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqe = D::Nqe;
     const DevTables& T = c_tab[P - 1];
     __shared__ double sVe[3 * Nqe * Np];
-    extern __shared__ double sU[];
+    extern __shared__ __align__(16) double sU[];
     for (int t = threadIdx.x; t < 3 * Nqe * Np; t += blockDim.x) {
         const int k = t / (Nqe * Np), r = t - k * (Nqe * Np), j = r / Np, i = r - j * Np;
         sVe[t] = T.Ve[k][j][i];
@@ -926,11 +968,11 @@ static void launch_stage_pf(int block, const StageArgs& a, hipStream_t s) {
         return;
     }
     if (a.staged) {
-        const size_t lds = (size_t)4 * Deg<P>::Np * a.lds_stride * sizeof(double);
-        static size_t lds_set = 65536;  // above 64 KiB the kernel must opt in
-        if (lds > lds_set) {
+        constexpr size_t lds = (size_t)4 * Deg<P>::Np * staged_stride(P) * sizeof(double);
+        static bool opted = false;  // above 64 KiB the kernel must opt in
+        if (lds > 65536 && !opted) {
             hipFuncSetAttribute((const void*)k_stage<P, FLUX, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            lds_set = lds;
+            opted = true;
         }
         hipLaunchKernelGGL((k_stage<P, FLUX, true>), grid, blk, lds, s, a);
         return;
