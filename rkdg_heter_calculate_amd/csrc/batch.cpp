@@ -28,7 +28,13 @@ struct rkdg_batch {
     int n = 0, P = 2, Np = 6, block = 256, persistent = 0, xcd_remap = 1;
     long long ld = 0;           // total padded elements
     std::vector<int> E, off;    // per instance
+    std::vector<std::vector<int32_t>> order;  // per instance: caller element at device position p
     std::vector<std::vector<double>> ex, ey;
+    int* nloc = nullptr;        // staged form tables (P2)
+    int* hoff = nullptr;
+    int* hids = nullptr;
+    bool staged_ok = false;     // the halo of every tile fits the staged LDS image
+    int staged = 0;             // staged form in use (chosen by the autotuner)
     double* U[3] = {nullptr, nullptr, nullptr};
     double* geo = nullptr;
     int* nbr = nullptr;
@@ -80,6 +86,11 @@ static StageArgs bargs(rkdg_batch* b, int stage, int slot) {
     a.Uout = b->U[u_out];
     a.geo = b->geo;
     a.nbr = b->nbr;
+    a.nloc = b->nloc;
+    a.hoff = b->hoff;
+    a.hids = b->hids;
+    a.staged = b->staged;
+    a.lds_stride = staged_stride(b->P);
     a.dtflag = b->dtflag;
     a.st = b->st;
     a.tile_mesh = b->tile_mesh;
@@ -143,6 +154,7 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     b->xcd_remap = 1;
     b->E.resize(n);
     b->off.resize(n);
+    b->order.resize(n);
     b->ex.resize(n);
     b->ey.resize(n);
     long long tot = 0;
@@ -178,20 +190,49 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
         std::vector<int32_t> se((size_t)3 * E);
         if (m.shared_edge) std::memcpy(se.data(), m.shared_edge, sizeof(int32_t) * 3 * E);
         else if (build_shared_edges(E, m.tri, m.neighbour, se.data(), merr)) break;
-        for (int e = 0; e < E; ++e) {
+        for (int e = 0; e < E; ++e)
             for (int k = 0; k < 3; ++k) {
                 int v = m.tri[3 * e + k];
+                if (v < 0 || v >= m.n_vertices) { merr = "vertex id out of range"; break; }
                 b->ex[i][3 * e + k] = m.vx[v];
                 b->ey[i][3 * e + k] = m.vy[v];
             }
+        if (!merr.empty()) break;
+        // device order: the caller's, or the Z-order of the barycentres (compact tiles)
+        std::vector<int32_t>& ord = b->order[i];
+        std::vector<int32_t> pos(E);
+        ord.resize(E);
+        if (m.order) {
+            for (int p = 0; p < E; ++p) ord[p] = m.order[p];
+        } else {
+            std::vector<double> bx(E), by(E);
+            for (int e = 0; e < E; ++e) {
+                bx[e] = (b->ex[i][3 * e] + b->ex[i][3 * e + 1] + b->ex[i][3 * e + 2]) / 3;
+                by[e] = (b->ey[i][3 * e] + b->ey[i][3 * e + 1] + b->ey[i][3 * e + 2]) / 3;
+            }
+            morton_order(E, bx.data(), by.data(), ord.data());
+        }
+        {
+            std::vector<char> seen(E, 0);
+            for (int p = 0; p < E; ++p) {
+                const int e = ord[p];
+                if (e < 0 || e >= E || seen[e]) { merr = "order is not a permutation"; break; }
+                seen[e] = 1;
+                pos[e] = p;
+            }
+        }
+        if (!merr.empty()) break;
+        for (int p = 0; p < E; ++p) {
+            const int e = ord[p];
             double g[GEO_ROWS];
             element_geometry(&b->ex[i][3 * e], &b->ey[i][3 * e], g);
             if (!(g[4] > 0)) { merr = "degenerate element"; break; }
-            for (int r = 0; r < GEO_ROWS; ++r) hgeo[(size_t)r * ld + off + e] = g[r];
+            for (int r = 0; r < GEO_ROWS; ++r) hgeo[(size_t)r * ld + off + p] = g[r];
             for (int k = 0; k < 3; ++k) {
                 int nb = m.neighbour[3 * e + k];
-                if (nb >= 0) hnbr[(size_t)k * ld + off + e] = ((off + nb) << 2) | se[3 * e + k];
-                else hnbr[(size_t)k * ld + off + e] = -(m.boundary ? m.boundary[3 * e + k] : RKDG_BC_OUTFLOW);
+                if (nb >= E) { merr = "neighbour id out of range"; break; }
+                if (nb >= 0) hnbr[(size_t)k * ld + off + p] = ((off + pos[nb]) << 2) | se[3 * e + k];
+                else hnbr[(size_t)k * ld + off + p] = -(m.boundary ? m.boundary[3 * e + k] : RKDG_BC_OUTFLOW);
             }
             int id = m.dt_id ? m.dt_id[e] : e;
             bool inc = true;
@@ -199,7 +240,7 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
                 int R = p0.reduction_threads > 0 ? p0.reduction_threads : 512;
                 if (R >= 128) inc = (id % 128) < 64;
             }
-            hflag[off + e] = inc;
+            hflag[off + p] = inc;
         }
     }
     if (!merr.empty()) { delete b; return bfail(nullptr, RKDG_ERR_MESH, merr); }
@@ -229,6 +270,24 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     BCHK(b, hipMemcpy(b->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice));
     BCHK(b, hipMemcpy(b->cfl, hcfl.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     b->bytes = 3 * ub + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size();
+    if (p0.precision == RKDG_PREC_FP64 && b->P == 2) {
+        // staged-form tables for 256-element tiles (instances start on 256-element
+        // boundaries, so no tile spans two instances)
+        std::vector<int> hoff, hids, hnloc;
+        const int B = staged_block(b->P);
+        const int hmax = build_halo(hnbr.data(), ld, (int)ld, B, hoff, hids, hnloc);
+        if (B + hmax <= staged_stride(b->P)) {
+            BCHK(b, hipMalloc(&b->nloc, sizeof(int) * hnloc.size()));
+            BCHK(b, hipMalloc(&b->hoff, sizeof(int) * hoff.size()));
+            BCHK(b, hipMalloc(&b->hids, sizeof(int) * std::max<size_t>(1, hids.size())));
+            BCHK(b, hipMemcpy(b->nloc, hnloc.data(), sizeof(int) * hnloc.size(), hipMemcpyHostToDevice));
+            BCHK(b, hipMemcpy(b->hoff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice));
+            if (!hids.empty())
+                BCHK(b, hipMemcpy(b->hids, hids.data(), sizeof(int) * hids.size(), hipMemcpyHostToDevice));
+            b->bytes += sizeof(int) * (hnloc.size() + hoff.size() + hids.size());
+            b->staged_ok = true;
+        }
+    }
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
@@ -251,6 +310,7 @@ int rkdg_batch_destroy(rkdg_batch* b) {
     if (b->graph2) hipGraphExecDestroy(b->graph2);
     for (int k = 0; k < 3; ++k) hipFree(b->U[k]);
     hipFree(b->geo); hipFree(b->nbr); hipFree(b->dtflag); hipFree(b->st); hipFree(b->cfl); hipFree(b->tile_mesh);
+    hipFree(b->nloc); hipFree(b->hoff); hipFree(b->hids);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return RKDG_OK;
@@ -263,12 +323,13 @@ int rkdg_batch_set_state(rkdg_batch* b, int32_t i, int32_t basis, const double* 
     std::vector<double> h((size_t)4 * Np * E);
     parallel_for(E, [&](int a0, int a1) {
         double in[4 * MAXNP], o[4 * MAXNP];
-        for (int e = a0; e < a1; ++e) {
+        for (int p = a0; p < a1; ++p) {
+            const int e = b->order[i][p];
             for (int v = 0; v < 4; ++v)
                 for (int k = 0; k < Np; ++k) in[v * Np + k] = U[((size_t)v * Np + k) * E + e];
             if (basis == RKDG_BASIS_ELEMENT) element_to_native(P, &b->ex[i][3 * e], &b->ey[i][3 * e], in, o);
             else std::memcpy(o, in, sizeof(double) * 4 * Np);
-            for (int r = 0; r < 4 * Np; ++r) h[(size_t)r * E + e] = o[r];
+            for (int r = 0; r < 4 * Np; ++r) h[(size_t)r * E + p] = o[r];
         }
     });
     // rows of E doubles into rows of ld doubles, starting at the instance offset
@@ -290,8 +351,9 @@ int rkdg_batch_get_state(rkdg_batch* b, int32_t i, int32_t basis, double* U) {
     BCHK(b, hipStreamSynchronize(b->stream));
     parallel_for(E, [&](int a0, int a1) {
         double in[4 * MAXNP], o[4 * MAXNP];
-        for (int e = a0; e < a1; ++e) {
-            for (int r = 0; r < 4 * Np; ++r) in[r] = h[(size_t)r * E + e];
+        for (int p = a0; p < a1; ++p) {
+            const int e = b->order[i][p];
+            for (int r = 0; r < 4 * Np; ++r) in[r] = h[(size_t)r * E + p];
             if (basis == RKDG_BASIS_ELEMENT) native_to_element(P, &b->ex[i][3 * e], &b->ey[i][3 * e], in, o);
             else std::memcpy(o, in, sizeof(double) * 4 * Np);
             for (int v = 0; v < 4; ++v)
@@ -398,9 +460,13 @@ int rkdg_batch_autotune(rkdg_batch* b, int32_t* best_block) {
     hipEvent_t e0, e1;
     BCHK(b, hipEventCreate(&e0));
     BCHK(b, hipEventCreate(&e1));
+    int bs = b->staged;
+    for (int st = 0; st <= (b->staged_ok ? 1 : 0); ++st)
     for (int c : cands)
         for (int pw : pers) {
             if (b->prm.precision == RKDG_PREC_FP32_MFMA && c != 64) continue;
+            if (st && c != staged_block(b->P)) continue;  // the staged form has fixed tiles
+            b->staged = st;
             b->block = c;
             b->persistent = pw ? cus * pw * 256 / c : 0;
             int rc = upload_tiles(b);
@@ -415,12 +481,13 @@ int rkdg_batch_autotune(rkdg_batch* b, int32_t* best_block) {
             BCHK(b, hipEventSynchronize(e1));
             float ms;
             BCHK(b, hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) { best = ms; bb = c; bp = b->persistent; }
+            if (ms < best) { best = ms; bb = c; bp = b->persistent; bs = st; }
         }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     b->block = bb;
     b->persistent = bp;
+    b->staged = bs;
     int rc = upload_tiles(b);
     if (rc) return rc;
     if (best_block) *best_block = bb;

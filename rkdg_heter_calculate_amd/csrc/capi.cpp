@@ -106,6 +106,8 @@ struct rkdg_ctx {
     int staged = 0;
     int dma = 0;                // software-pipelined persistent form
     int persistent = 0;         // workgroups of the persistent grid (0: one per tile)
+    int stagger = 0;            // start offset of half of a persistent grid (8128-cycle sleeps)
+    int pipe = 0;               // staged form: persistent, next image overlapped with the update
     unsigned char* dtflag = nullptr;
     double* lim = nullptr;
     int* pairs = nullptr;
@@ -155,6 +157,8 @@ static StageArgs make_args(rkdg_ctx* c, int stage, int slot, int dt_mode, double
     a.staged = c->staged;
     a.dma = c->dma;
     a.persistent = c->persistent;
+    a.stagger = c->stagger;
+    a.pipe = c->pipe;
     a.dtflag = c->dtflag;
     a.st = c->st;
     a.ld = c->ld;
@@ -412,12 +416,21 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     }
     // staged form: per-tile halos, used when the tile's LDS image fits
     std::vector<int> hoff, hids, hnloc;
-    if (!c->dma && prm->precision == RKDG_PREC_FP64 && c->block == staged_block(c->P) && env_int("RKDG_STAGED", 1)) {
+    // (measured faster than the global-gather form at P1 and P2)
+    if (!c->dma && prm->precision == RKDG_PREC_FP64 && c->block == staged_block(c->P) &&
+        env_int("RKDG_STAGED", c->P <= 2 ? 1 : 0)) {
         const int hmax = build_halo(hnbr.data(), ld, E, c->block, hoff, hids, hnloc);
         const long long lds = (long long)4 * c->Np * staged_stride(c->P) * (long long)sizeof(double);
         if (c->block + hmax <= staged_stride(c->P) && lds <= max_lds_per_block(prm->device)) {
             c->staged = 1;
             c->lds_stride = staged_stride(c->P);
+            int cus = 0;
+            c->pipe = env_int("RKDG_PIPE", 0);
+            const int per_cu = env_int("RKDG_PERSIST", c->pipe ? 2 : 0);
+            if (per_cu > 0 && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, prm->device) == hipSuccess)
+                c->persistent = per_cu * cus;
+            if (c->pipe && c->persistent == 0) c->pipe = 0;
+            c->stagger = env_int("RKDG_STAGGER", 0);
         }
     }
 

@@ -144,6 +144,10 @@ struct StageArgs {
     int lds_stride;        // >= block + largest halo
     int staged;
     int dma;               // software-pipelined persistent form (single mesh, P <= 2, 256 lanes)
+    int pipe;              // staged form, persistent: next tile's image loaded during the update
+    int stagger;           // persistent grids: the second half of the workgroups starts this
+                           // many 8128-cycle sleeps late, so that the two workgroups sharing a
+                           // CU alternate their memory and arithmetic phases
 };
 
 enum { DTM_DEVICE = 0, DTM_HOST = 1, DTM_FIXED = 2 };

@@ -234,7 +234,9 @@ struct ElemGeo {
 };
 
 // Neighbour sources of the element core.
-enum { NB_GLOBAL = 0, NB_LDS = 1 };
+// NB_LDS_LEAN reads the LDS image like NB_LDS but evaluates one edge point at a time
+// (fewer live registers, less instruction-level parallelism).
+enum { NB_GLOBAL = 0, NB_LDS = 1, NB_LDS_LEAN = 2 };
 
 // Residual, SSP-RK3 update and store of one element, from its modes Uo (already in
 // registers), its geometry and its three neighbour codes.  Neighbour traces come from
@@ -242,18 +244,26 @@ enum { NB_GLOBAL = 0, NB_LDS = 1 };
 // stages 1 and 2 is then prefetched into the same registers during the last edge) or
 // from the tile's LDS image (NB_LDS, slots of a.nloc).  Stage 2 folds the element's CFL
 // wave speed and residual into ws_l, r0_l, r1_l (running maxima).
+// Work registers of one element between its residual and its update.
+template <int P>
+struct ElemWork {
+    double R[4][Deg<P>::Np];   // residual (mass not yet applied)
+    double Nb[4][Deg<P>::Np];  // neighbour modes; with prefetch, the step-start modes afterwards
+    double per;                // perimeter
+};
+
 template <int P, int FLUX, int NBSRC>
-__device__ __forceinline__ void element_core(const StageArgs& a, int e, const double (&Uo)[4][Deg<P>::Np],
-                                             const ElemGeo& eg, int w0, int w1, int w2, double dt, bool dtf,
-                                             const double* sVe, const double* sU, DevState* st, double& ws_l,
-                                             double& r0_l, double& r1_l) {
+__device__ __forceinline__ void element_residual(const StageArgs& a, int e, const double (&Uo)[4][Deg<P>::Np],
+                                                 const ElemGeo& eg, int w0, int w1, int w2, const double* sVe,
+                                                 const double* sU, ElemWork<P>& W) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqv = D::Nqv, Nqe = D::Nqe;
     constexpr bool PF = NBSRC == NB_GLOBAL && Prefetch<P>::on;
     const DevTables& T = c_tab[P - 1];
     const long long ld = a.ld;
     const double g = a.gamma;
-    double R[4][Np], Nb[4][Np];
+    auto& R = W.R;
+    auto& Nb = W.Nb;
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
@@ -294,7 +304,7 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
         const int w = k == 0 ? w0 : (k == 1 ? w1 : w2);
         const double lk = lk_n, nx = nx_n, ny = ny_n;
         per += lk;
-        if (NBSRC == NB_LDS && k < 2) {
+        if (NBSRC != NB_GLOBAL && k < 2) {
             // next edge's geometry, in flight during this edge
             lk_n = soa(a.geo, 6 + k, ld, e);
             nx_n = soa(a.geo, 10 + 2 * k, ld, e);
@@ -303,7 +313,7 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
         // neighbour modes: LDS slot, registers (prefetched), or a load now
         const double* src = nullptr;
         if (w >= 0) {
-            if (NBSRC == NB_LDS) src = sU + (w >> 2);
+            if (NBSRC != NB_GLOBAL) src = sU + (w >> 2);
             else if (!PF) load_modes<Np>(a.U, ld, w >> 2, Nb);
         }
         constexpr int S = staged_stride(P);
@@ -314,7 +324,7 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
         double mR[4];
         if (w >= 0) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) mR[v] = NBSRC == NB_LDS ? src[(v * Np) * S] : Nb[v][0];
+            for (int v = 0; v < 4; ++v) mR[v] = NBSRC != NB_GLOBAL ? src[(v * Np) * S] : Nb[v][0];
         } else if (w == -RKDG_BC_FARFIELD) {
             // kernel_boundaryCondition CELL_FARFIELD (src/kernelfunctions.cu:184-201)
 #pragma unroll
@@ -416,9 +426,9 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
                     const double* t = tb + (Nqe - 1 - j) * Np;
     #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        double s = (NBSRC == NB_LDS ? src[(v * Np) * S] : Nb[v][0]) * t[0];
+                        double s = (NBSRC != NB_GLOBAL ? src[(v * Np) * S] : Nb[v][0]) * t[0];
     #pragma unroll
-                        for (int i = 1; i < Np; ++i) s = fma(NBSRC == NB_LDS ? src[(v * Np + i) * S] : Nb[v][i], t[i], s);
+                        for (int i = 1; i < Np; ++i) s = fma(NBSRC != NB_GLOBAL ? src[(v * Np + i) * S] : Nb[v][i], t[i], s);
                         uR[v] = s;
                     }
                 } else if (w == -RKDG_BC_FARFIELD) {
@@ -455,7 +465,7 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
         // state), issued once this edge no longer needs the registers
         if (k < 2) {
             const int wn = k == 0 ? w1 : w2;
-            if (NBSRC != NB_LDS) {
+            if (NBSRC == NB_GLOBAL) {
                 lk_n = soa(a.geo, 6 + k, ld, e);
                 nx_n = soa(a.geo, 10 + 2 * k, ld, e);
                 ny_n = soa(a.geo, 11 + 2 * k, ld, e);
@@ -465,8 +475,28 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
             load_modes<Np>(a.U0, ld, e, Nb);
         }
     }
+    W.per = per;
+}
 
+// SSP-RK3 combination, mass division and store of one element (stage 2 also folds its
+// CFL wave speed and residual into ws_l, r0_l, r1_l, running maxima).
+template <int P, int NBSRC>
+__device__ __forceinline__ void element_update(const StageArgs& a, int e, const double (&Uo)[4][Deg<P>::Np],
+                                               const ElemGeo& eg, ElemWork<P>& W, double dt, bool dtf,
+                                               DevState* st, double& ws_l, double& r0_l, double& r1_l) {
+    constexpr int Np = Deg<P>::Np;
+    constexpr bool PF = NBSRC == NB_GLOBAL && Prefetch<P>::on;
+    const DevTables& T = c_tab[P - 1];
+    const long long ld = a.ld;
+    const double g = a.gamma;
+    auto& R = W.R;
+    auto& Nb = W.Nb;
+    const double per = W.per;
     // ---- SSP-RK3 stage (src/kernelfunctions.cu:640-748) ---------------------------
+    // The step-start modes are all loaded before the first store: U0 and Uout are the same
+    // buffer in stage 2, so a load placed after a store could not be moved ahead of it and
+    // every mode would cost a full memory round trip.
+    if (!PF && a.stage > 0) load_modes<Np>(a.U0, ld, e, Nb);
     const double ia = frcp(eg.area);
     double mean[4];
 #pragma unroll
@@ -478,7 +508,7 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
             if (a.stage == 0) {
                 out = Uo[v][i] + dt * r;
             } else {
-                const double o0 = PF ? Nb[v][i] : soa(a.U0, v * Np + i, ld, e);
+                const double o0 = Nb[v][i];
                 if (a.stage == 1) {
                     out = 0.75 * o0 + 0.25 * Uo[v][i] + 0.25 * dt * r;
                 } else {
@@ -505,6 +535,17 @@ __device__ __forceinline__ void element_core(const StageArgs& a, int e, const do
     }
 }
 
+
+template <int P, int FLUX, int NBSRC>
+__device__ __forceinline__ void element_core(const StageArgs& a, int e, const double (&Uo)[4][Deg<P>::Np],
+                                             const ElemGeo& eg, int w0, int w1, int w2, double dt, bool dtf,
+                                             const double* sVe, const double* sU, DevState* st, double& ws_l,
+                                             double& r0_l, double& r1_l) {
+    ElemWork<P> W;
+    element_residual<P, FLUX, NBSRC>(a, e, Uo, eg, w0, w1, w2, sVe, sU, W);
+    element_update<P, NBSRC>(a, e, Uo, eg, W, dt, dtf, st, ws_l, r0_l, r1_l);
+}
+
 // Per-mesh bookkeeping done once per stage by the workgroup owning the mesh's first tile:
 // stage 0 fixes the step's dt and clears the reductions, stage 2 advances the time.
 __device__ __forceinline__ void stage_bookkeeping(const StageArgs& a, const TileInfo& ti, double dt, bool done) {
@@ -526,6 +567,83 @@ __device__ __forceinline__ void stage_bookkeeping(const StageArgs& a, const Tile
 // One workgroup tile, operands loaded from global memory.  STAGED: the tile's own modes
 // and its halo (the out-of-tile neighbours, listed per tile on the host) are first copied
 // into LDS, and every neighbour trace is evaluated from LDS.
+// Operands an element needs before its first edge: neighbour codes, geometry, dt flag.
+struct ElemOps {
+    int w0, w1, w2;
+    ElemGeo eg;
+    bool dtf;
+};
+
+__device__ __forceinline__ void load_element_ops(const StageArgs& a, const int* nb_ids, int e, ElemOps& o) {
+    const long long ld = a.ld;
+    o.w0 = soa(nb_ids, 0, ld, e);
+    o.w1 = soa(nb_ids, 1, ld, e);
+    o.w2 = soa(nb_ids, 2, ld, e);
+    o.eg.area = soa(a.geo, 4, ld, e);
+    o.eg.xix = soa(a.geo, 0, ld, e);
+    o.eg.xiy = soa(a.geo, 1, ld, e);
+    o.eg.etx = soa(a.geo, 2, ld, e);
+    o.eg.ety = soa(a.geo, 3, ld, e);
+    o.eg.lk0 = soa(a.geo, 5, ld, e);
+    o.eg.nx0 = soa(a.geo, 8, ld, e);
+    o.eg.ny0 = soa(a.geo, 9, ld, e);
+    o.dtf = a.stage == 2 && a.dtflag[e] != 0;
+}
+
+// Staged form.  The LDS image of a tile (its own modes, then its halo: the out-of-tile
+// neighbours listed per tile on the host) is filled by the direct-to-LDS load engine: the
+// own rows as 16-byte lanes (1 KiB per wave instruction), the halo as 4-byte lanes (two
+// lanes per double, 32 halo elements per instruction).  No data passes through registers.
+template <int P>
+struct Staged {
+    static constexpr int Np = Deg<P>::Np, S = staged_stride(P), B = staged_block(P), NR = 4 * Np;
+    static constexpr int NW = B / 64, PARTS = B / 128, HCH = (S - B + 31) / 32;
+};
+
+// This lane's halo ids of a tile (element hb * 32 + lane / 2 of the halo, chunk hb).
+template <int P>
+__device__ __forceinline__ int staged_halo_ids(const StageArgs& a, int tile, int (&hid)[Staged<P>::HCH]) {
+    const int lane = threadIdx.x & 63;
+    const int h0 = a.hoff[tile], nh = a.hoff[tile + 1] - h0;
+#pragma unroll
+    for (int hb = 0; hb < Staged<P>::HCH; ++hb) {
+        const int h = hb * 32 + (lane >> 1);
+        hid[hb] = h < nh ? a.hids[h0 + h] : 0;
+    }
+    return nh;
+}
+
+// Issues this wave's share of the direct loads of a tile's image.
+template <int P>
+__device__ __forceinline__ void staged_fill(const StageArgs& a, int tile, int nh, const int (&hid)[Staged<P>::HCH],
+                                            double* sU) {
+    using L = Staged<P>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long ld = a.ld;
+    char* lds = (char*)sU;
+    const long long e0 = (long long)tile * L::B;
+    for (int c = wave; c < L::NR * L::PARTS; c += L::NW) {
+        const int r = c / L::PARTS, part = c - r * L::PARTS;
+        const char* src = (const char*)(a.U + r * ld + e0) + part * 1024 + lane * 16;
+        __builtin_amdgcn_global_load_lds((const void*)src,
+                                         (__attribute__((address_space(3))) void*)(lds + r * L::S * 8 + part * 1024), 16, 0, 0);
+    }
+#pragma unroll
+    for (int hb = 0; hb < L::HCH; ++hb) {
+        if (hb * 32 >= nh) break;  // uniform across the workgroup
+        const bool on = hb * 32 + (lane >> 1) < nh;
+        const char* src0 = (const char*)a.U + (size_t)hid[hb] * 8 + (lane & 1) * 4;
+        for (int r = wave; r < L::NR; r += L::NW) {
+            if (on)
+                __builtin_amdgcn_global_load_lds((const void*)(src0 + (size_t)r * ld * 8),
+                                                 (__attribute__((address_space(3))) void*)(lds + (r * L::S + L::B + hb * 32) * 8),
+                                                 4, 0, 0);
+        }
+    }
+}
+
+// One workgroup tile, operands loaded from global memory (STAGED: neighbour traces from
+// the tile's LDS image).
 template <int P, int FLUX, bool STAGED>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& ti, const double* sVe, double* sU) {
     constexpr int Np = Deg<P>::Np;
@@ -537,82 +655,99 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
     const long long ld = a.ld;
     const int e = ti.tile * blockDim.x + threadIdx.x;
     double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
-
-    const int* nb_ids = STAGED ? a.nloc : a.nbr;
     double Uo[4][Np];
-    int w0 = 0, w1 = 0, w2 = 0;
-    ElemGeo eg{};
-    bool dtf = false;
-    auto load_element = [&]() {
-        w0 = soa(nb_ids, 0, ld, e);
-        w1 = soa(nb_ids, 1, ld, e);
-        w2 = soa(nb_ids, 2, ld, e);
-        eg.area = soa(a.geo, 4, ld, e);
-        eg.xix = soa(a.geo, 0, ld, e);
-        eg.xiy = soa(a.geo, 1, ld, e);
-        eg.etx = soa(a.geo, 2, ld, e);
-        eg.ety = soa(a.geo, 3, ld, e);
-        eg.lk0 = soa(a.geo, 5, ld, e);
-        eg.nx0 = soa(a.geo, 8, ld, e);
-        eg.ny0 = soa(a.geo, 9, ld, e);
-        dtf = a.stage == 2 && a.dtflag[e] != 0;
-    };
+    ElemOps o{};
     if (STAGED) {
-        // The LDS image is filled by the direct-to-LDS load engine in one batch: the own
-        // rows as 16-byte lanes (1 KiB per wave instruction), the halo as 4-byte lanes (two
-        // lanes per double, 32 halo elements per instruction).  No data passes through
-        // registers, and the element's own operands are loaded in the same round trip.
-        constexpr int S = staged_stride(P), B = staged_block(P), NR = 4 * Np;
-        constexpr int NW = B / 64, PARTS = B / 128, HCH = (S - B + 31) / 32;
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        char* lds = (char*)sU;
-        const int h0 = a.hoff[ti.tile], nh = a.hoff[ti.tile + 1] - h0;
-        int hid[HCH];
-#pragma unroll
-        for (int hb = 0; hb < HCH; ++hb) {
-            const int h = hb * 32 + (lane >> 1);
-            hid[hb] = h < nh ? a.hids[h0 + h] : 0;
-        }
-        const long long e0 = (long long)ti.tile * B;
-        for (int c = wave; c < NR * PARTS; c += NW) {
-            const int r = c / PARTS, part = c - r * PARTS;
-            const char* src = (const char*)(a.U + r * ld + e0) + part * 1024 + lane * 16;
-            __builtin_amdgcn_global_load_lds((const void*)src,
-                                             (__attribute__((address_space(3))) void*)(lds + r * S * 8 + part * 1024), 16, 0, 0);
-        }
-        if (e < a.E) load_element();
-        for (int c = wave; c < NR * HCH; c += NW) {
-            const int r = c / HCH, hb = c - r * HCH;
-            if (hb * 32 >= nh) continue;  // uniform across the wave
-            int id = hid[0];
-#pragma unroll
-            for (int q = 1; q < HCH; ++q) id = hb == q ? hid[q] : id;
-            if (hb * 32 + (lane >> 1) < nh) {
-                const char* src = (const char*)(a.U + r * ld) + (size_t)id * 8 + (lane & 1) * 4;
-                __builtin_amdgcn_global_load_lds((const void*)src,
-                                                 (__attribute__((address_space(3))) void*)(lds + (r * S + B + hb * 32) * 8),
-                                                 4, 0, 0);
-            }
-        }
+        int hid[Staged<P>::HCH];
+        const int nh = staged_halo_ids<P>(a, ti.tile, hid);
+        staged_fill<P>(a, ti.tile, nh, hid, sU);
+        if (e < a.E) load_element_ops(a, a.nloc, e, o);  // same round trip as the image
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int i = 0; i < Np; ++i) Uo[v][i] = sU[(v * Np + i) * S + threadIdx.x];
+            for (int i = 0; i < Np; ++i) Uo[v][i] = sU[(v * Np + i) * Staged<P>::S + threadIdx.x];
     } else if (e < a.E) {
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
             for (int i = 0; i < Np; ++i) Uo[v][i] = soa(a.U, v * Np + i, ld, e);
-        load_element();
+        load_element_ops(a, a.nbr, e, o);
     }
 
     if (e < a.E)
-        element_core<P, FLUX, STAGED ? NB_LDS : NB_GLOBAL>(a, e, Uo, eg, w0, w1, w2, dt, dtf, sVe, sU, ti.st, ws_l,
-                                                          r0_l, r1_l);
+        element_core<P, FLUX, STAGED ? NB_LDS : NB_GLOBAL>(a, e, Uo, o.eg, o.w0, o.w1, o.w2, dt, o.dtf, sVe, sU,
+                                                          ti.st, ws_l, r0_l, r1_l);
     if (a.stage == 2)
         block_max3_atomic(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], &ti.st->res[0], &ti.st->res[1], true);
+}
+
+// Persistent staged form with the next tile's image in flight during the current tile's
+// SSP-RK3 update: once every wave has finished its residual (a barrier), the LDS image is
+// free, the direct loads of the next tile's image and its operands are issued, and only
+// then are the step-start modes read and the results stored.  The image's memory traffic
+// thus overlaps the update's instead of waiting for it.
+template <int P, int FLUX>
+__global__ void __launch_bounds__(256) k_stage_pipe(StageArgs a) {
+    using D = Deg<P>;
+    using L = Staged<P>;
+    constexpr int Np = D::Np, Nqe = D::Nqe;
+    const DevTables& T = c_tab[P - 1];
+    __shared__ double sVe[3 * Nqe * Np];
+    extern __shared__ __align__(16) double sU[];
+    for (int t = threadIdx.x; t < 3 * Nqe * Np; t += blockDim.x) {
+        const int k = t / (Nqe * Np), r = t - k * (Nqe * Np), j = r / Np, i = r - j * Np;
+        sVe[t] = T.Ve[k][j][i];
+    }
+    const int nit = iter_count(a);
+    const int G = gridDim.x, tid = threadIdx.x;
+    int it = blockIdx.x;
+    TileInfo ti;
+    bool have = it < nit && tile_of_iter(a, it, nit, ti);
+    ElemOps o{};
+    if (have) {
+        int hid[L::HCH];
+        const int nh = staged_halo_ids<P>(a, ti.tile, hid);
+        staged_fill<P>(a, ti.tile, nh, hid, sU);
+        const int e = ti.tile * L::B + tid;
+        if (e < a.E) load_element_ops(a, a.nloc, e, o);
+    }
+    while (have) {
+        bool done;
+        const double dt = stage_dt(a, ti, done);
+        stage_bookkeeping(a, ti, dt, done);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();  // the tile's image and operands have landed (and sVe on entry)
+        const int e = ti.tile * L::B + tid;
+        const bool live = e < a.E && !done;
+        double Uo[4][Np];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 0; i < Np; ++i) Uo[v][i] = sU[(v * Np + i) * L::S + tid];
+        it += G;
+        TileInfo tn;
+        const bool have_n = it < nit && tile_of_iter(a, it, nit, tn);
+        ElemWork<P> W;
+        if (live) element_residual<P, FLUX, NB_LDS_LEAN>(a, e, Uo, o.eg, o.w0, o.w1, o.w2, sVe, sU, W);
+        __syncthreads();  // every wave is done with the image
+        ElemOps on{};
+        if (have_n) {
+            int hid[L::HCH];
+            const int nh = staged_halo_ids<P>(a, tn.tile, hid);
+            staged_fill<P>(a, tn.tile, nh, hid, sU);
+            const int en = tn.tile * L::B + tid;
+            if (en < a.E) load_element_ops(a, a.nloc, en, on);
+        }
+        double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
+        if (live) element_update<P, NB_LDS_LEAN>(a, e, Uo, o.eg, W, dt, o.dtf, ti.st, ws_l, r0_l, r1_l);
+        if (a.stage == 2)
+            block_max3_atomic(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], &ti.st->res[0], &ti.st->res[1], true);
+        ti = tn;
+        o = on;
+        have = have_n;
+    }
 }
 
 // One workgroup per tile, operands from global memory (STAGED: neighbours from the
@@ -630,6 +765,8 @@ __global__ void __launch_bounds__(256) k_stage(StageArgs a) {
         sVe[t] = T.Ve[k][j][i];
     }
     __syncthreads();
+    if (a.stagger > 0 && blockIdx.x >= (gridDim.x >> 1))
+        for (int i = 0; i < a.stagger; ++i) __builtin_amdgcn_s_sleep(127);
     const int nit = iter_count(a);
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         TileInfo ti;
@@ -969,12 +1106,14 @@ static void launch_stage_pf(int block, const StageArgs& a, hipStream_t s) {
     }
     if (a.staged) {
         constexpr size_t lds = (size_t)4 * Deg<P>::Np * staged_stride(P) * sizeof(double);
-        static bool opted = false;  // above 64 KiB the kernel must opt in
+        static bool opted = false;  // above 64 KiB the kernels must opt in
         if (lds > 65536 && !opted) {
             hipFuncSetAttribute((const void*)k_stage<P, FLUX, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipFuncSetAttribute((const void*)k_stage_pipe<P, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             opted = true;
         }
-        hipLaunchKernelGGL((k_stage<P, FLUX, true>), grid, blk, lds, s, a);
+        if (a.pipe) hipLaunchKernelGGL((k_stage_pipe<P, FLUX>), grid, blk, lds, s, a);
+        else hipLaunchKernelGGL((k_stage<P, FLUX, true>), grid, blk, lds, s, a);
         return;
     }
     hipLaunchKernelGGL((k_stage<P, FLUX, false>), grid, blk, 0, s, a);
