@@ -24,6 +24,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes_per_element_stage(Np):
     # U read + U written + step-start state read in 2 of 3 stages, 14 geometry doubles,
     # 3 neighbour ids; neighbour re-reads counted as cache hits
@@ -53,7 +57,7 @@ def large_p2_point(n, steps):
     return out
 
 
-def batch_point(count, steps):
+def batch_point(count, steps, precision="fp64"):
     """Batched sweep: `count` periodic vortex instances of uneven size in one launch per stage."""
     import ctypes
     import numpy as np
@@ -68,7 +72,7 @@ def batch_point(count, steps):
     params = (L.Params * count)()
     for i in range(count):
         meshes[i] = views[i]
-        params[i] = make_params(degree=2, flux="lf", dt_policy="max", cfl=inst[i][2])
+        params[i] = make_params(degree=2, flux="lf", dt_policy="max", cfl=inst[i][2], precision=precision)
     h = ctypes.c_void_p()
     L.check(lib.rkdg_batch_create(count, meshes, params, ctypes.byref(h)), None, "batch_create")
     for i in range(count):
@@ -88,7 +92,7 @@ def batch_point(count, steps):
     E = int(info[1])
     return {"instances": count, "elements": E, "steps": steps, "block": int(best.value),
             "value": E * 6 * 4 * 3 * steps / dt, "ms_per_step": dt * 1e3 / steps,
-            "note": "FP64 batched form, hipGraph-replayed stages, autotuned workgroup size"}
+            "note": f"{precision} batched form, hipGraph-replayed stages, autotuned tiles and persistence"}
 
 
 def precision_point(g, P, steps):
@@ -135,6 +139,85 @@ def precision_point(g, P, steps):
     return out
 
 
+def _rel_l2(a, b, area):
+    """Relative L2 difference per conserved variable of two native-basis states."""
+    import numpy as np
+    return [float(np.sqrt(np.sum(area * np.sum((b[v] - a[v]) ** 2, axis=0)) / np.sum(area * np.sum(a[v] ** 2, axis=0))))
+            for v in range(4)]
+
+
+def _areas(g):
+    import numpy as np
+    ex, ey = g.element_coords()
+    return 0.5 * np.abs((ex[:, 1] - ex[:, 0]) * (ey[:, 2] - ey[:, 0]) - (ex[:, 2] - ex[:, 0]) * (ey[:, 1] - ey[:, 0]))
+
+
+def riemann_point(n=1024, t_end=0.3):
+    """Four-quadrant Riemann problem, P1, Roe, TVB M=50 (+ positivity), FP64 and the
+    precision form (limiter in FP64), each run to t_end from the same cell averages."""
+    from rkdg_heter_calculate_amd import Solver, IC_RIEMANN4
+    from rkdg_heter_calculate_amd import _lib as L
+    from rkdg_heter_calculate_amd.workloads import riemann_p1
+    g, kw, ic, _ = riemann_p1(n)
+    out, states = {"elements": g.n_elements, "t_end": t_end}, {}
+    for prec in ("fp64", "fp32"):
+        s = Solver(g, precision=prec, **kw)
+        s.project_ic(IC_RIEMANN4, ic)
+        s.sync()
+        t0 = time.perf_counter()
+        nsteps, t = 0, 0.0
+        while t < t_end:
+            n, t = s.advance(5000, t_end=t_end)
+            nsteps += n
+            progress(f"riemann {prec}: t = {t:.4f} after {nsteps} steps")
+            if n == 0:
+                break
+        s.sync()
+        wall = time.perf_counter() - t0
+        states[prec] = s.get_state(L.BASIS_NATIVE)
+        progress(f"riemann {prec}: {nsteps} steps to t = {t:.4f} in {wall:.1f} s")
+        out[prec] = {"steps": nsteps, "t": t, "wall_s": wall,
+                     "value": g.n_elements * 3 * 4 * 3 * nsteps / wall, "ms_per_step": wall * 1e3 / nsteps,
+                     "limited_cells_last_stage": int((s.flags() != 0).sum())}
+        s.close()
+    out["rel_l2_fp32_vs_fp64"] = _rel_l2(states["fp64"], states["fp32"], _areas(g))
+    out["note"] = "value counts limiter launches inside the step; wall time includes one host check per 32 steps"
+    return out
+
+
+def p3_point(n=2048, steps=50):
+    """Large P3 vortex (Roe, CFL 0.2): FP64 and the precision form, stage-kernel roofline and
+    the precision form's difference after the same fixed-dt steps."""
+    from rkdg_heter_calculate_amd import Solver, IC_VORTEX, vortex_params
+    from rkdg_heter_calculate_amd import _lib as L
+    from rkdg_heter_calculate_amd.workloads import vortex_p3
+    g, kw, vp, _ = vortex_p3(n)
+    E = g.n_elements
+    out = {"elements": E, "steps": steps}
+    B = algorithmic_bytes_per_element_stage(10)
+    dt0 = None
+    states = {}
+    for prec in ("fp64", "fp32"):
+        s = Solver(g, precision=prec, **kw)
+        s.project_ic(IC_VORTEX, vp)
+        if dt0 is None:
+            dt0 = s.compute_dt()
+        s.advance(2)
+        r = s.time_advance(steps)
+        progress(f"p3 {prec}: {r['total_ms'] / steps:.2f} ms per step")
+        out[prec] = {"ms_per_step": r["total_ms"] / steps, "stage_ms": r["stage_ms"],
+                     "value": E * 10 * 4 * 3 * steps / (r["total_ms"] / 1e3),
+                     "achieved_GBs": B * E / (r["stage_ms"] / 1e3) / 1e9}
+        s.close()
+        f = Solver(g, precision=prec, dt_policy="fixed", dt_fixed=dt0, degree=3, flux="roe")
+        f.project_ic(IC_VORTEX, vp)
+        f.advance(steps)
+        states[prec] = f.get_state(L.BASIS_NATIVE)
+        f.close()
+    out["rel_l2_fp32_vs_fp64"] = _rel_l2(states["fp64"], states["fp32"], _areas(g))
+    return out
+
+
 def cpu_baseline(g, P, steps, threads):
     """The oracle (CPU restatement of the reference algorithm) on a bounded sample."""
     from oracle import pyoracle as po
@@ -175,7 +258,10 @@ def main():
     ap.add_argument("--degree", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=50, help="oracle steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the 2.1M-triangle and batched legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip every extra leg")
+    ap.add_argument("--extra", default="p2_2M,batch",
+                    help="comma-separated extra legs: p2_2M, batch, batch_fp32, riemann, p3 "
+                         "(riemann runs 2 x ~25k steps to t = 0.3 and p3 builds an 8.4M-triangle mesh)")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-MFMA leg")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batched leg")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the stream-copy measurement")
@@ -193,10 +279,12 @@ def main():
     E = g.n_elements
     s = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=0.3)
     s.project_ic(IC_VORTEX, vortex_params())
+    progress(f"{E} elements set up; {args.warmup} warm-up steps")
     s.advance(args.warmup)
     s.sync()
     t0 = time.perf_counter()
     r = s.time_advance(args.steps)
+    progress(f"timed {args.steps} steps: {r['total_ms'] / args.steps:.3f} ms per step")
     s.sync()
     wall = time.perf_counter() - t0
     t_sim, nsteps = s.time()
@@ -214,6 +302,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         v, secs = cpu_baseline(g, P, args.cpu_steps, threads)
+        progress(f"cpu baseline: {args.cpu_steps} oracle steps in {secs:.1f} s")
         cpu = {"value": v, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} SSP-RK3 steps of the same {E}-triangle P{P} mesh "
                          f"({secs:.1f} s, OpenMP over elements)"}
@@ -229,11 +318,23 @@ def main():
     prec = None
     if not args.no_precision:
         prec = precision_point(g, P, args.steps)
+        progress("precision form leg done")
 
     extra = {}
     if not args.no_extra:
-        extra["p2_2M_fp64"] = large_p2_point(1024, 20)
-        extra["batch_fp64"] = batch_point(args.batch, 100)
+        legs = [x for x in args.extra.split(",") if x]
+        runners = {"p2_2M": ("p2_2M_fp64", lambda: large_p2_point(1024, 20)),
+                   "batch": ("batch_fp64", lambda: batch_point(args.batch, 100)),
+                   "batch_fp32": ("batch_fp32_mfma", lambda: batch_point(args.batch, 100, precision="fp32")),
+                   "riemann": ("riemann_p1_2M", lambda: riemann_point(1024, 0.3)),
+                   "p3": ("vortex_p3_8M", lambda: p3_point(2048, 50))}
+        for leg in legs:
+            if leg not in runners:
+                sys.exit(f"bench.py: unknown extra leg {leg!r}")
+            key, fn = runners[leg]
+            t0 = time.perf_counter()
+            extra[key] = fn()
+            progress(f"extra leg {leg} done in {time.perf_counter() - t0:.1f} s")
 
     out = {
         "metric": "DOF-stage updates/s on 1 MI355X, FP64 vs FP32-MFMA at matched L2 error",

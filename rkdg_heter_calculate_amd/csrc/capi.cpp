@@ -644,6 +644,13 @@ static int build_graph(rkdg_ctx* c) {
 static int advance_impl(rkdg_ctx* c, int nsteps, double t_end, int* done_out, double* t_out) {
     int use_graph = env_int("RKDG_GRAPH", 1) && t_end <= 0;
     int done = 0;
+    long long nsteps0 = 0;  // the device's step counter before this call
+    if (done_out) {
+        DevState h0;
+        HIPCHK(c, hipMemcpyAsync(&h0, c->st, sizeof h0, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        nsteps0 = h0.nsteps;
+    }
     if (t_end > 0) {
         // chunks without host synchronisation inside; the end time turns the remaining
         // launches of a chunk into no-ops on the device
@@ -690,7 +697,7 @@ static int advance_impl(rkdg_ctx* c, int nsteps, double t_end, int* done_out, do
         DevState h;
         HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        if (done_out) *done_out = (int)h.nsteps;
+        if (done_out) *done_out = (int)(h.nsteps - nsteps0);  // steps taken (the end time stops it)
         if (t_out) *t_out = h.t[c->steps & 1];
         if (h.err_pos != INT_MAX) return check_physical(c);
     }

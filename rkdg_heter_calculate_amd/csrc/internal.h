@@ -169,7 +169,7 @@ struct LimArgs {
 // stride is a compile-time constant so that every row is an immediate LDS offset (below
 // 64 KiB: 4 Np rows of stride doubles).
 constexpr int staged_block(int P) { return P <= 2 ? 256 : 128; }
-constexpr int staged_stride(int P) { return P <= 2 ? 336 : 200; }
+constexpr int staged_stride(int P) { return P <= 2 ? 352 : 200; }
 
 int upload_tables(int P, const DevTables& t);  // per current device
 int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream);

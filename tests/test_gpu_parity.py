@@ -130,6 +130,23 @@ def test_run_twice_bitwise():
     assert np.array_equal(a, b)
 
 
+def test_advance_counts_steps_per_call():
+    print("case: advance step counts and end time", flush=True)
+    g = vortex_grid(16, jitter=0.2, diagonal=1, seed=5)
+    s = Solver(g, degree=2, dt_policy="fixed", dt_fixed=1e-3)
+    s.project_ic(1, vortex_params())
+    n1, t1 = s.advance(5)
+    n2, t2 = s.advance(5)
+    assert (n1, n2) == (5, 5)
+    assert abs(t1 - 5e-3) < 1e-15 and abs(t2 - 1e-2) < 1e-15
+    # an end time inside the next steps stops the run there, the last step clipped
+    n3, t3 = s.advance(50, t_end=0.0125)
+    assert n3 == 3 and abs(t3 - 0.0125) < 1e-15
+    n4, t4 = s.advance(10, t_end=0.0125)
+    assert n4 == 0 and t4 == t3
+    s.close()
+
+
 def test_batch_equals_single():
     print("case: batch vs single", flush=True)
     import ctypes
