@@ -173,7 +173,7 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     std::vector<double> hU((size_t)4 * b->Np * ld, 0.0);
     // dummy padding cell: unit right triangle, fluid at rest
     {
-        double x[3] = {0, 1, 0}, y[3] = {0, 0, 1}, g[GEO_ROWS];
+        double x[3] = {0, 1, 0}, y[3] = {0, 0, 1}, g[GEO_HOST];
         element_geometry(x, y, g);
         for (long long p = 0; p < ld; ++p) {
             for (int r = 0; r < GEO_ROWS; ++r) hgeo[(size_t)r * ld + p] = g[r];
@@ -224,9 +224,9 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
         if (!merr.empty()) break;
         for (int p = 0; p < E; ++p) {
             const int e = ord[p];
-            double g[GEO_ROWS];
+            double g[GEO_HOST];
             element_geometry(&b->ex[i][3 * e], &b->ey[i][3 * e], g);
-            if (!(g[4] > 0)) { merr = "degenerate element"; break; }
+            if (!element_ccw(g)) { merr = "degenerate or clockwise element"; break; }
             for (int r = 0; r < GEO_ROWS; ++r) hgeo[(size_t)r * ld + off + p] = g[r];
             for (int k = 0; k < 3; ++k) {
                 int nb = m.neighbour[3 * e + k];

@@ -4,7 +4,7 @@
 //   U[3]    three FP64 state buffers [4 vars][Np modes][ld]: the step-start state and
 //           the two intermediate SSP-RK3 stages (neighbours read the previous stage while
 //           the next is written, so stages cannot update in place)
-//   geo     [14][ld] inverse Jacobian, area, edge lengths, outward normals
+//   geo     [5][ld] inverse Jacobian and area (edge lengths and normals follow from them)
 //   nbr     [3][ld] int32 packed (device position << 2 | neighbour's edge), or -tag
 //   dtflag  [ld] uint8, 1 if the element takes part in the CFL reduction
 //   lim     [7][ld], pairs [ld] (TVB limiter geometry, only with the limiter)
@@ -104,7 +104,6 @@ struct rkdg_ctx {
     int* hids = nullptr;        // staged form: halo element positions
     int lds_stride = 0;         // staged form: doubles per LDS row (block + largest halo)
     int staged = 0;
-    int dma = 0;                // software-pipelined persistent form
     int persistent = 0;         // workgroups of the persistent grid (0: one per tile)
     int stagger = 0;            // start offset of half of a persistent grid (8128-cycle sleeps)
     int pipe = 0;               // staged form: persistent, next image overlapped with the update
@@ -155,7 +154,6 @@ static StageArgs make_args(rkdg_ctx* c, int stage, int slot, int dt_mode, double
     a.hids = c->hids;
     a.lds_stride = c->lds_stride;
     a.staged = c->staged;
-    a.dma = c->dma;
     a.persistent = c->persistent;
     a.stagger = c->stagger;
     a.pipe = c->pipe;
@@ -340,9 +338,9 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     std::string merr;
     for (int p = 0; p < E; ++p) {
         int e = c->order[p];
-        double g[GEO_ROWS];
+        double g[GEO_HOST];
         element_geometry(&c->ex[3 * e], &c->ey[3 * e], g);
-        if (!(g[4] > 0)) { merr = "degenerate element " + std::to_string(e); break; }
+        if (!element_ccw(g)) { merr = "degenerate or clockwise element " + std::to_string(e); break; }
         for (int r = 0; r < GEO_ROWS; ++r) hgeo[(size_t)r * ld + p] = g[r];
         for (int k = 0; k < 3; ++k) {
             int n = m->neighbour[3 * e + k];
@@ -405,19 +403,10 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
         }
     }
 
-    // software-pipelined persistent form for P1/P2 on 256-lane tiles: two workgroups per
-    // CU (its LDS buffer is 44-69 KiB)
-    if (prm->precision == RKDG_PREC_FP64 && c->P <= 2 && c->block == 256 && env_int("RKDG_DMA", 0)) {
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, prm->device) == hipSuccess && cus > 0) {
-            c->dma = 1;
-            c->persistent = 2 * cus;
-        }
-    }
     // staged form: per-tile halos, used when the tile's LDS image fits
     std::vector<int> hoff, hids, hnloc;
     // (measured faster than the global-gather form at P1 and P2)
-    if (!c->dma && prm->precision == RKDG_PREC_FP64 && c->block == staged_block(c->P) &&
+    if (prm->precision == RKDG_PREC_FP64 && c->block == staged_block(c->P) &&
         env_int("RKDG_STAGED", c->P <= 2 ? 1 : 0)) {
         const int hmax = build_halo(hnbr.data(), ld, E, c->block, hoff, hids, hnloc);
         const long long lds = (long long)4 * c->Np * staged_stride(c->P) * (long long)sizeof(double);

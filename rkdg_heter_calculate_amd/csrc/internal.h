@@ -13,7 +13,11 @@ constexpr int MAXNP = 10;   // P3
 constexpr int MAXQV = 12;
 constexpr int MAXQE = 4;
 constexpr int NVAR = 4;
-constexpr int GEO_ROWS = 14;  // xi_x, xi_y, eta_x, eta_y, area, l0, l1, l2, n0x, n0y, n1x, n1y, n2x, n2y
+// Host geometry of one element: xi_x, xi_y, eta_x, eta_y, area, l0, l1, l2, n0x, n0y, n1x,
+// n1y, n2x, n2y.  The device keeps the first GEO_ROWS; the kernels derive the edge lengths
+// and outward normals from the inverse Jacobian and the area.
+constexpr int GEO_HOST = 14;
+constexpr int GEO_ROWS = 5;
 constexpr int LIM_ROWS = 7;   // 3 x (a1, a2), h^2
 
 inline int np_of(int P) { return (P + 1) * (P + 2) / 2; }
@@ -67,8 +71,10 @@ struct TriRule { int n; std::vector<double> xi, eta, w; };
 const TriRule& projection_rule();
 const std::vector<double>& projection_psi(int P);  // psi_j at the rule's points, [q][j]
 
-// Element geometry rows for one element (GEO_ROWS doubles)
+// Element geometry rows for one element (GEO_HOST doubles)
 void element_geometry(const double* x, const double* y, double* g);
+// True for a non-degenerate counter-clockwise element (positive Jacobian determinant)
+inline bool element_ccw(const double* g) { return g[4] > 0 && g[0] * g[3] - g[1] * g[2] > 0; }
 
 // The reference's per-element Gram-Schmidt basis (P <= 3), evaluated in physical
 // coordinates.  Built from monomials 1, X, Y, X^2, XY, Y^2, X^3, ... with
@@ -143,7 +149,6 @@ struct StageArgs {
     const int* hids;       // halo element positions, slot blockDim.x + h of tile t is hids[hoff[t] + h]
     int lds_stride;        // >= block + largest halo
     int staged;
-    int dma;               // software-pipelined persistent form (single mesh, P <= 2, 256 lanes)
     int pipe;              // staged form, persistent: next tile's image loaded during the update
     int stagger;           // persistent grids: the second half of the workgroups starts this
                            // many 8128-cycle sleeps late, so that the two workgroups sharing a

@@ -230,8 +230,26 @@ __device__ __forceinline__ void load_modes(const double* __restrict__ U, long lo
 
 // Geometry an element needs before its first edge (the rest is loaded one edge ahead).
 struct ElemGeo {
-    double xix, xiy, etx, ety, area, lk0, nx0, ny0;
+    double xix, xiy, etx, ety, area;
 };
+
+// Edge k's outward unit normal and length.  For the affine map of a CCW triangle (edge k
+// opposite vertex k) the outward normal times the length is n_k l_k = 2A s_k with
+// s_0 = xi + eta, s_1 = -xi, s_2 = -eta (xi = (xi_x, xi_y), eta = (eta_x, eta_y): the rows
+// of the inverse Jacobian, 2A its determinant); so n_k = s_k / |s_k|, l_k = 2A |s_k|.
+__device__ __forceinline__ void edge_frame(double sx, double sy, double area, double& nx, double& ny, double& lk) {
+    const double q = sqrt(sx * sx + sy * sy);
+    const double iq = frcp(q);
+    nx = sx * iq;
+    ny = sy * iq;
+    lk = 2.0 * area * q;
+}
+
+__device__ __forceinline__ void edge_normal(const ElemGeo& eg, int k, double& nx, double& ny, double& lk) {
+    const double sx = k == 0 ? eg.xix + eg.etx : (k == 1 ? -eg.xix : -eg.etx);
+    const double sy = k == 0 ? eg.xiy + eg.ety : (k == 1 ? -eg.xiy : -eg.ety);
+    edge_frame(sx, sy, eg.area, nx, ny, lk);
+}
 
 // Neighbour sources of the element core.
 // NB_LDS_LEAN reads the LDS image like NB_LDS but evaluates one edge point at a time
@@ -298,18 +316,20 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
 
     // ---- surface integral, element-owned -----------------------------------------
     // Both loops stay rolled: one quadrature point's traces and flux are live at a time.
-    double lk_n = eg.lk0, nx_n = eg.nx0, ny_n = eg.ny0, per = 0.0;
+    // Edge k's direction s_k (edge_normal) for k = 1, 2 is re-read one edge ahead (L2 hits)
+    // rather than keeping the inverse Jacobian live across the loop.
+    double sx_n = eg.xix + eg.etx, sy_n = eg.xiy + eg.ety, per = 0.0;
 #pragma unroll 1
     for (int k = 0; k < 3; ++k) {
         const int w = k == 0 ? w0 : (k == 1 ? w1 : w2);
-        const double lk = lk_n, nx = nx_n, ny = ny_n;
-        per += lk;
-        if (NBSRC != NB_GLOBAL && k < 2) {
-            // next edge's geometry, in flight during this edge
-            lk_n = soa(a.geo, 6 + k, ld, e);
-            nx_n = soa(a.geo, 10 + 2 * k, ld, e);
-            ny_n = soa(a.geo, 11 + 2 * k, ld, e);
+        const double sx = sx_n, sy = sy_n;
+        if (k < 2) {
+            sx_n = -soa(a.geo, 2 * k, ld, e);
+            sy_n = -soa(a.geo, 2 * k + 1, ld, e);
         }
+        double nx, ny, lk;
+        edge_frame(sx, sy, eg.area, nx, ny, lk);
+        per += lk;
         // neighbour modes: LDS slot, registers (prefetched), or a load now
         const double* src = nullptr;
         if (w >= 0) {
@@ -461,15 +481,10 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
                 }
             }
         }
-        // next edge's geometry and neighbour modes (after the last edge: the step-start
-        // state), issued once this edge no longer needs the registers
+        // next edge's neighbour modes (after the last edge: the step-start state), issued
+        // once this edge no longer needs the registers
         if (k < 2) {
             const int wn = k == 0 ? w1 : w2;
-            if (NBSRC == NB_GLOBAL) {
-                lk_n = soa(a.geo, 6 + k, ld, e);
-                nx_n = soa(a.geo, 10 + 2 * k, ld, e);
-                ny_n = soa(a.geo, 11 + 2 * k, ld, e);
-            }
             if (PF && wn >= 0) load_modes<Np>(a.U, ld, wn >> 2, Nb);
         } else if (PF && a.stage > 0) {
             load_modes<Np>(a.U0, ld, e, Nb);
@@ -584,9 +599,6 @@ __device__ __forceinline__ void load_element_ops(const StageArgs& a, const int* 
     o.eg.xiy = soa(a.geo, 1, ld, e);
     o.eg.etx = soa(a.geo, 2, ld, e);
     o.eg.ety = soa(a.geo, 3, ld, e);
-    o.eg.lk0 = soa(a.geo, 5, ld, e);
-    o.eg.nx0 = soa(a.geo, 8, ld, e);
-    o.eg.ny0 = soa(a.geo, 9, ld, e);
     o.dtf = a.stage == 2 && a.dtflag[e] != 0;
 }
 
@@ -754,7 +766,7 @@ __global__ void __launch_bounds__(256) k_stage_pipe(StageArgs a) {
 // tile's LDS image, passed as dynamic shared memory of 4 Np rows of staged_stride(P)
 // doubles; the workgroup then has staged_block(P) lanes).
 template <int P, int FLUX, bool STAGED>
-__global__ void __launch_bounds__(256) k_stage(StageArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED && P <= 2 ? 2 : 1))) k_stage(StageArgs a) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqe = D::Nqe;
     const DevTables& T = c_tab[P - 1];
@@ -774,116 +786,6 @@ __global__ void __launch_bounds__(256) k_stage(StageArgs a) {
         stage_tile<P, FLUX, STAGED>(a, ti, sVe, sU);
         __syncthreads();  // LDS and the reduction scratch are reused by the next tile
     }
-}
-
-// ---- software-pipelined persistent form -----------------------------------------------
-// Each workgroup (256 lanes, tiles of 256 elements) walks its tiles in order.  While it
-// computes tile t from registers, the LDS direct-load engine (global_load_lds_dwordx4,
-// no registers involved) brings tile t+1's modes, the geometry used before the first edge
-// and the neighbour codes into one LDS buffer; neighbours are gathered from global memory
-// one edge ahead and the step-start state during the last edge.  Memory traffic thus
-// overlaps the arithmetic inside every wave instead of relying on other waves.
-template <int P>
-struct DmaLayout {
-    static constexpr int Np = Deg<P>::Np;
-    static constexpr int NR = 4 * Np;      // modal rows
-    static constexpr int NG = 8;           // geometry rows 0-5, 8, 9
-    static constexpr int TILE = 256;
-    static constexpr int CHUNKS = 2 * (NR + NG) + 3;   // 1 KiB per wave-wide 16-byte load
-    static constexpr size_t BYTES = (size_t)(NR + NG) * TILE * 8 + 3 * TILE * 4;
-};
-
-__device__ __forceinline__ int dma_geo_row(int gi) { return gi < 6 ? gi : gi + 2; }
-
-// Issues this wave's share of the LDS direct loads of tile t.
-template <int P>
-__device__ __forceinline__ void dma_issue(const StageArgs& a, int t, char* buf) {
-    using L = DmaLayout<P>;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const long long ld = a.ld;
-    const long long e0 = (long long)t * L::TILE;
-    for (int c = wave; c < L::CHUNKS; c += nwave) {
-        const char* src;
-        int dst;
-        if (c < 2 * L::NR) {
-            const int r = c >> 1, h = c & 1;
-            src = (const char*)(a.U + r * ld + e0) + h * 1024;
-            dst = r * 2048 + h * 1024;
-        } else if (c < 2 * (L::NR + L::NG)) {
-            const int gi = (c - 2 * L::NR) >> 1, h = c & 1;
-            src = (const char*)(a.geo + dma_geo_row(gi) * ld + e0) + h * 1024;
-            dst = (L::NR + gi) * 2048 + h * 1024;
-        } else {
-            const int k = c - 2 * (L::NR + L::NG);
-            src = (const char*)(a.nbr + k * ld + e0);
-            dst = (L::NR + L::NG) * 2048 + k * 1024;
-        }
-        __builtin_amdgcn_global_load_lds((const void*)(src + lane * 16),
-                                         (__attribute__((address_space(3))) void*)(buf + dst), 16, 0, 0);
-    }
-}
-
-template <int P, int FLUX>
-__global__ void __launch_bounds__(256) k_stage_dma(StageArgs a) {
-    using D = Deg<P>;
-    using L = DmaLayout<P>;
-    constexpr int Np = D::Np, Nqe = D::Nqe;
-    const DevTables& T = c_tab[P - 1];
-    __shared__ double sVe[3 * Nqe * Np];
-    extern __shared__ double dbuf[];
-    char* buf = (char*)dbuf;
-    for (int t = threadIdx.x; t < 3 * Nqe * Np; t += blockDim.x) {
-        const int k = t / (Nqe * Np), r = t - k * (Nqe * Np), j = r / Np, i = r - j * Np;
-        sVe[t] = T.Ve[k][j][i];
-    }
-    const int nit = iter_count(a);
-    const int G = gridDim.x;
-    const int tid = threadIdx.x;
-    double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
-    DevState* st = a.st;
-
-    int it = blockIdx.x;
-    TileInfo ti;
-    bool have = it < nit && tile_of_iter(a, it, nit, ti);
-    if (have) dma_issue<P>(a, ti.tile, buf);
-    while (have) {
-        bool done;
-        const double dt = stage_dt(a, ti, done);
-        stage_bookkeeping(a, ti, dt, done);
-        // this tile's LDS image is complete once every wave's direct loads have landed
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        const int e = ti.tile * L::TILE + tid;
-        double Uo[4][Np];
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int i = 0; i < Np; ++i) Uo[v][i] = dbuf[(v * Np + i) * L::TILE + tid];
-        ElemGeo eg;
-        const double* gb = dbuf + L::NR * L::TILE;
-        eg.xix = gb[0 * L::TILE + tid];
-        eg.xiy = gb[1 * L::TILE + tid];
-        eg.etx = gb[2 * L::TILE + tid];
-        eg.ety = gb[3 * L::TILE + tid];
-        eg.area = gb[4 * L::TILE + tid];
-        eg.lk0 = gb[5 * L::TILE + tid];
-        eg.nx0 = gb[6 * L::TILE + tid];
-        eg.ny0 = gb[7 * L::TILE + tid];
-        const int* nb = (const int*)(buf + (size_t)(L::NR + L::NG) * L::TILE * 8);
-        const int w0 = nb[tid], w1 = nb[L::TILE + tid], w2 = nb[2 * L::TILE + tid];
-        __syncthreads();  // the buffer is free: start the next tile's loads
-        it += G;
-        TileInfo tn;
-        const bool have_n = it < nit && tile_of_iter(a, it, nit, tn);
-        if (have_n) dma_issue<P>(a, tn.tile, buf);
-        if (!done && e < a.E) {
-            const bool dtf = a.stage == 2 && a.dtflag[e] != 0;
-            element_core<P, FLUX, NB_GLOBAL>(a, e, Uo, eg, w0, w1, w2, dt, dtf, sVe, nullptr, st, ws_l, r0_l, r1_l);
-        }
-        ti = tn;
-        have = have_n;
-    }
-    if (a.stage == 2) block_max3_atomic(ws_l, r0_l, r1_l, &st->ws[1 - a.slot], &st->res[0], &st->res[1], true);
 }
 
 #include "stage_mx.inc"
@@ -906,8 +808,14 @@ __global__ void __launch_bounds__(256) k_wavespeed(StageArgs a) {
             const double p = (g - 1.0) * (a.U[(3 * Np) * ld + e] - 0.5 * rho * (u * u + v * v));
             if (!(rho > 0.0) || !(p > 0.0)) atomicMin(&ti.st->err_pos, e);
             const double c = sqrt(g * p / rho);
-            const double area = a.geo[4 * ld + e];
-            const double s = (sqrt(u * u + v * v) + c) * (a.geo[5 * ld + e] + a.geo[6 * ld + e] + a.geo[7 * ld + e]) / area;
+            const ElemGeo eg{a.geo[0 * ld + e], a.geo[1 * ld + e], a.geo[2 * ld + e], a.geo[3 * ld + e], a.geo[4 * ld + e]};
+            double per = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                double nx, ny, lk;
+                edge_normal(eg, k, nx, ny, lk);
+                per += lk;
+            }
+            const double s = (sqrt(u * u + v * v) + c) * per / eg.area;
             if (a.dtflag[e] && s > 0.0) s_l = s;
         }
         block_max3_atomic(s_l, 0.0, 0.0, &ti.st->ws[a.slot], nullptr, nullptr, false);
@@ -1093,17 +1001,6 @@ static int grid_blocks(const StageArgs& a) {
 template <int P, int FLUX>
 static void launch_stage_pf(int block, const StageArgs& a, hipStream_t s) {
     dim3 grid(grid_blocks(a)), blk(block);
-    if (a.dma && P <= 2) {
-        constexpr size_t lds = DmaLayout<P <= 2 ? P : 2>::BYTES;
-        static bool opted = false;  // above 64 KiB the kernel must opt in
-        if (!opted) {
-            hipFuncSetAttribute((const void*)k_stage_dma<P <= 2 ? P : 2, FLUX>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            opted = true;
-        }
-        hipLaunchKernelGGL((k_stage_dma<P <= 2 ? P : 2, FLUX>), grid, blk, lds, s, a);
-        return;
-    }
     if (a.staged) {
         constexpr size_t lds = (size_t)4 * Deg<P>::Np * staged_stride(P) * sizeof(double);
         static bool opted = false;  // above 64 KiB the kernels must opt in
