@@ -160,6 +160,13 @@ void build_ref_basis(int P, RefBasis& B) {
             for (int i = 0; i < Np; ++i) T.fVv[nq + k * ne + j][i] = (float)T.Ve[k][j][i];
     for (int i = 0; i < Np; ++i)
         for (int p = 0; p < 16; ++p) T.fTest[i][p] = T.fVv[p][i];
+    for (int q = 0; q < nq; ++q) {
+        T.fwv[q] = (float)T.wv[q];
+        for (int i = 0; i < Np; ++i) {
+            T.fDxi[q][i] = (float)T.Dxi[q][i];
+            T.fDeta[q][i] = (float)T.Deta[q][i];
+        }
+    }
 }
 
 }  // namespace

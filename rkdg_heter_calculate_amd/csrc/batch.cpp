@@ -270,7 +270,7 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     BCHK(b, hipMemcpy(b->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice));
     BCHK(b, hipMemcpy(b->cfl, hcfl.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     b->bytes = 3 * ub + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size();
-    if (p0.precision == RKDG_PREC_FP64 && b->P == 2) {
+    if (b->P <= 2) {
         // staged-form tables for 256-element tiles (instances start on 256-element
         // boundaries, so no tile spans two instances)
         std::vector<int> hoff, hids, hnloc;
@@ -287,6 +287,12 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
             b->bytes += sizeof(int) * (hnloc.size() + hoff.size() + hids.size());
             b->staged_ok = true;
         }
+    }
+    // precision form: staged tiles, one lane per element, when every halo fits (else four
+    // lanes per element on 64-element tiles)
+    if (p0.precision == RKDG_PREC_FP32_MFMA && b->staged_ok) {
+        b->block = staged_block(b->P);
+        b->staged = 1;
     }
     int cus = 256;
     hipDeviceProp_t prop;
@@ -464,7 +470,7 @@ int rkdg_batch_autotune(rkdg_batch* b, int32_t* best_block) {
     for (int st = 0; st <= (b->staged_ok ? 1 : 0); ++st)
     for (int c : cands)
         for (int pw : pers) {
-            if (b->prm.precision == RKDG_PREC_FP32_MFMA && c != 64) continue;
+            if (b->prm.precision == RKDG_PREC_FP32_MFMA && !st && c != 64) continue;
             if (st && c != staged_block(b->P)) continue;  // the staged form has fixed tiles
             b->staged = st;
             b->block = c;

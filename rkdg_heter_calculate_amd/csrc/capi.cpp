@@ -273,7 +273,10 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     c->P = prm->degree; c->Np = np_of(c->P); c->Nqv = nqv_of(c->P); c->Nqe = nqe_of(c->P);
     c->E = m->n_elements;
     c->block = prm->block > 0 ? prm->block : env_int("RKDG_BLOCK", prm->degree >= 3 ? 128 : 256);
-    if (prm->precision == RKDG_PREC_FP32_MFMA) c->block = 64;  // 64 elements on 256 threads
+    // precision form: staged tiles of 256 elements at P1/P2 (lane per element), else 64
+    // elements on 256 threads (four lanes per element)
+    const bool px = prm->precision == RKDG_PREC_FP32_MFMA && c->P <= 2 && env_int("RKDG_PX", 1);
+    if (prm->precision == RKDG_PREC_FP32_MFMA) c->block = px ? staged_block(c->P) : 64;
     if (c->block % 64 || c->block < 64 || c->block > 256) { delete c; return fail(nullptr, RKDG_ERR_ARG, "block must be 64..256, multiple of 64"); }
     c->xcd_remap = env_int("RKDG_XCD_REMAP", 1);
     c->ntiles = (c->E + c->block - 1) / c->block;
@@ -406,7 +409,7 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     // staged form: per-tile halos, used when the tile's LDS image fits
     std::vector<int> hoff, hids, hnloc;
     // (measured faster than the global-gather form at P1 and P2)
-    if (prm->precision == RKDG_PREC_FP64 && c->block == staged_block(c->P) &&
+    if ((prm->precision == RKDG_PREC_FP64 || px) && c->block == staged_block(c->P) &&
         env_int("RKDG_STAGED", c->P <= 2 ? 1 : 0)) {
         const int hmax = build_halo(hnbr.data(), ld, E, c->block, hoff, hids, hnloc);
         const long long lds = (long long)4 * c->Np * staged_stride(c->P) * (long long)sizeof(double);
@@ -421,6 +424,10 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
             if (c->pipe && c->persistent == 0) c->pipe = 0;
             c->stagger = env_int("RKDG_STAGGER", 0);
         }
+    }
+    if (px && !c->staged) {  // a halo did not fit: four lanes per element
+        c->block = 64;
+        c->ntiles = (c->E + c->block - 1) / c->block;
     }
 
     // device

@@ -40,6 +40,9 @@ struct DevTables {
     // single-precision copies for the MFMA form
     float fVv[16][16];             // [point (vol 0..Nqv-1, then 3*Nqe edge)][mode], zero padded
     float fTest[16][16];           // [mode][point] test functions (gradients handled separately)
+    float fDxi[MAXQV][MAXNP];      // single-precision gradients at the volume points
+    float fDeta[MAXQV][MAXNP];
+    float fwv[MAXQV];
 };
 
 // Device-side step state: wave-speed maxima and times per step parity.

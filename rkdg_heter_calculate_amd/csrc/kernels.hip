@@ -789,6 +789,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED
 }
 
 #include "stage_mx.inc"
+#include "stage_px.inc"
 
 // CFL wave speed of the current state, max-reduced into ws[slot]
 template <int P>
@@ -1029,9 +1030,36 @@ static void launch_stage_mx_p(int flux, const StageArgs& a, hipStream_t s) {
     else hipLaunchKernelGGL((k_stage_mx<P, RKDG_FLUX_LF>), grid, blk, 0, s, a);
 }
 
+template <int P, int FLUX>
+static void launch_stage_px_pf(const StageArgs& a, hipStream_t s) {
+    constexpr size_t lds = (size_t)4 * Deg<P>::Np * staged_stride(P) * sizeof(double);
+    static bool opted = false;  // above 64 KiB the kernel must opt in
+    if (lds > 65536 && !opted) {
+        hipFuncSetAttribute((const void*)k_stage_px<P, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        opted = true;
+    }
+    hipLaunchKernelGGL((k_stage_px<P, FLUX>), dim3(grid_blocks(a)), dim3(staged_block(P)), lds, s, a);
+}
+
+template <int P>
+static void launch_stage_px_p(int flux, const StageArgs& a, hipStream_t s) {
+    if (flux == RKDG_FLUX_ROE) launch_stage_px_pf<P, RKDG_FLUX_ROE>(a, s);
+    else launch_stage_px_pf<P, RKDG_FLUX_LF>(a, s);
+}
+
 int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (prec == RKDG_PREC_FP32_MFMA) {
+        // staged tiles of 256 elements, one lane per element (P1, P2)
+        if (a.staged) {
+            if (block != 256) return (int)hipErrorInvalidValue;
+            switch (P) {
+                case 1: launch_stage_px_p<1>(flux, a, s); break;
+                case 2: launch_stage_px_p<2>(flux, a, s); break;
+                default: return (int)hipErrorInvalidValue;
+            }
+            return (int)hipGetLastError();
+        }
         // four lanes per element: tiles of 64 elements on 256 threads
         if (block != 64) return (int)hipErrorInvalidValue;
         switch (P) {

@@ -236,3 +236,43 @@ def test_freestream_state_matches_reference_formula():
     ut = np.sqrt(1.4) * 0.4
     al = 5.0 * np.arctan(1.0) * 4 / 180
     assert np.allclose(fs, [1.0, ut * np.cos(al), ut * np.sin(al), 0.5 * ut * ut + 1 / 0.4], rtol=1e-15)
+
+
+def test_edge_normals_follow_from_inverse_jacobian():
+    """The stage kernels derive edge k's outward normal times its length as 2A s_k with
+    s_0 = xi + eta, s_1 = -xi, s_2 = -eta (xi, eta: rows of the inverse Jacobian of the
+    affine map, 2A its determinant).  Checked against the vertex formulas of
+    src/triangleinfo.cpp:171-189 (edge k opposite vertex k, CCW) on random triangles."""
+    rng = np.random.default_rng(5)
+    n = 0
+    while n < 300:
+        x, y = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        det = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0])
+        if abs(det) < 1e-2:
+            continue
+        if det < 0:
+            x[[1, 2]], y[[1, 2]] = x[[2, 1]], y[[2, 1]]
+            det = -det
+        n += 1
+        xi = np.array([y[2] - y[0], -(x[2] - x[0])]) / det
+        eta = np.array([-(y[1] - y[0]), x[1] - x[0]]) / det
+        area = 0.5 * det
+        s = [xi + eta, -xi, -eta]
+        nl = [np.array([y[2] - y[1], x[1] - x[2]]), np.array([y[0] - y[2], x[2] - x[0]]),
+              np.array([y[1] - y[0], x[0] - x[1]])]
+        for k in range(3):
+            assert np.allclose(2 * area * s[k], nl[k], rtol=1e-12, atol=1e-13), (k, x, y)
+            # outward: the normal points away from the opposite vertex
+            mid = 0.5 * np.array([x[(k + 1) % 3] + x[(k + 2) % 3], y[(k + 1) % 3] + y[(k + 2) % 3]])
+            assert np.dot(s[k], mid - np.array([x[k], y[k]])) > 0
+
+
+def test_create_rejects_clockwise_elements():
+    """Because the kernels derive normals from the inverse Jacobian, rkdg_create refuses an
+    element given clockwise (the mesh contract of rkdg.h) before touching the device."""
+    from rkdg_heter_calculate_amd import Solver
+    g = Grid.splitquad(4, 4, periodic=True)
+    g.tri[5] = g.tri[5][[0, 2, 1]]
+    with pytest.raises(L.RkdgError) as ei:
+        Solver(g, degree=1)
+    assert ei.value.code == L.ERR_MESH and "clockwise" in str(ei.value)
