@@ -29,9 +29,16 @@ def progress(msg):
 
 
 def algorithmic_bytes_per_element_stage(Np):
-    # U read + U written + step-start state read in 2 of 3 stages, 14 geometry doubles,
-    # 3 neighbour ids; neighbour re-reads counted as cache hits
+    # The roofline yardstick: U read + U written + step-start state read in 2 of 3 stages,
+    # the reference's 14 geometry doubles per element, 3 neighbour ids; neighbour re-reads
+    # counted as cache hits
     return 8 * 4 * Np * (1 + 1 + 2.0 / 3.0) + 112 + 12
+
+
+def moved_bytes_per_element_stage(Np):
+    # What the stage kernels read and write: the same state traffic, 5 geometry doubles
+    # (inverse Jacobian and area; edge normals and lengths are derived), 3 neighbour ids
+    return 8 * 4 * Np * (1 + 1 + 2.0 / 3.0) + 40 + 12
 
 
 def workload(n=512, P=2):
@@ -263,6 +270,9 @@ def main():
                     help="comma-separated extra legs: p2_2M, batch, batch_fp32, riemann, p3 "
                          "(riemann runs 2 x ~25k steps to t = 0.3 and p3 builds an 8.4M-triangle mesh)")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-MFMA leg")
+    ap.add_argument("--form", choices=("fp64", "fp32"), default="fp64",
+                    help="form of the main timed leg (fp32: the FP32-MFMA precision form; its "
+                         "comparison leg is then skipped)")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batched leg")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the stream-copy measurement")
     args = ap.parse_args()
@@ -277,9 +287,10 @@ def main():
     Np = (P + 1) * (P + 2) // 2
     g = workload(args.n, P)
     E = g.n_elements
-    s = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=0.3)
+    fp32 = args.form == "fp32"
+    s = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=0.3, precision=args.form)
     s.project_ic(IC_VORTEX, vortex_params())
-    progress(f"{E} elements set up; {args.warmup} warm-up steps")
+    progress(f"{E} elements set up ({args.form}); {args.warmup} warm-up steps")
     s.advance(args.warmup)
     s.sync()
     t0 = time.perf_counter()
@@ -296,7 +307,8 @@ def main():
     B = algorithmic_bytes_per_element_stage(Np)
     per_launch = B * E
     achieved = per_launch / (r["stage_ms"] / 1e3) / 1e9
-    traffic = read_traffic(f"k_stage_p{P}_lf_{args.n}x{args.n}_hbm.md")
+    kname = f"k_stage_px_p{P}" if fp32 else f"k_stage_p{P}"
+    traffic = read_traffic(f"{kname}_lf_{args.n}x{args.n}_hbm.md")
 
     cpu = None
     if not args.no_cpu:
@@ -316,7 +328,7 @@ def main():
         copy_gbs = gbs.value
 
     prec = None
-    if not args.no_precision:
+    if not args.no_precision and not fp32:
         prec = precision_point(g, P, args.steps)
         progress("precision form leg done")
 
@@ -347,14 +359,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if fp32 else "f64",
         "data": "synthetic (seeded jittered split-quad mesh, projected isentropic vortex)",
         "config": {"workload": f"isentropic vortex, periodic [0,10]^2, {args.n}x{args.n} jittered split quads "
-                               f"({E} triangles), P{P}, LF, SSP-RK3 CFL 0.3, FP64",
-                   "elements": E, "degree": P, "dofs": dofs, "form": "plain FP64"},
+                               f"({E} triangles), P{P}, LF, SSP-RK3 CFL 0.3, " + ("FP32-MFMA" if fp32 else "FP64"),
+                   "elements": E, "degree": P, "dofs": dofs,
+                   "form": "precision (FP32-MFMA contractions, FP64 state)" if fp32 else "plain FP64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"k_stage<{P},LF>", "bytes_per_launch": per_launch,
+                     "kernel": f"k_stage_px<{P},LF>" if fp32 else f"k_stage<{P},LF>", "bytes_per_launch": per_launch,
+                     "bytes_moved_per_launch": moved_bytes_per_element_stage(Np) * E,
                      "avg_launch_ms": r["stage_ms"], "measured_copy_GBs": copy_gbs},
         "cpu_baseline": cpu,
         "precision_form": prec,
