@@ -256,17 +256,21 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
         rkdg_batch_destroy(b);
         return bfail(nullptr, RKDG_ERR_ARG, "batch too large (state or geometry array above 4 GiB)");
     }
-    for (int k = 0; k < 3; ++k) {
-        BCHK(b, hipMalloc(&b->U[k], ub));
-        BCHK(b, hipMemcpy(b->U[k], hU.data(), ub, hipMemcpyHostToDevice));
+    // host images are [rows][ld]; the device layout is tile-major
+    {
+        const std::vector<double> tU = to_tile_major(hU, 4 * b->Np, ld);
+        for (int k = 0; k < 3; ++k) {
+            BCHK(b, hipMalloc(&b->U[k], ub));
+            BCHK(b, hipMemcpy(b->U[k], tU.data(), ub, hipMemcpyHostToDevice));
+        }
     }
     BCHK(b, hipMalloc(&b->geo, sizeof(double) * hgeo.size()));
     BCHK(b, hipMalloc(&b->nbr, sizeof(int) * hnbr.size()));
     BCHK(b, hipMalloc(&b->dtflag, hflag.size()));
     BCHK(b, hipMalloc(&b->st, sizeof(DevState) * n));
     BCHK(b, hipMalloc(&b->cfl, sizeof(double) * n));
-    BCHK(b, hipMemcpy(b->geo, hgeo.data(), sizeof(double) * hgeo.size(), hipMemcpyHostToDevice));
-    BCHK(b, hipMemcpy(b->nbr, hnbr.data(), sizeof(int) * hnbr.size(), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpy(b->geo, to_tile_major(hgeo, GEO_ROWS, ld).data(), sizeof(double) * hgeo.size(), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpy(b->nbr, to_tile_major(hnbr, NBR_ROWS, ld).data(), sizeof(int) * hnbr.size(), hipMemcpyHostToDevice));
     BCHK(b, hipMemcpy(b->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice));
     BCHK(b, hipMemcpy(b->cfl, hcfl.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     b->bytes = 3 * ub + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size();
@@ -280,7 +284,8 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
             BCHK(b, hipMalloc(&b->nloc, sizeof(int) * hnloc.size()));
             BCHK(b, hipMalloc(&b->hoff, sizeof(int) * hoff.size()));
             BCHK(b, hipMalloc(&b->hids, sizeof(int) * std::max<size_t>(1, hids.size())));
-            BCHK(b, hipMemcpy(b->nloc, hnloc.data(), sizeof(int) * hnloc.size(), hipMemcpyHostToDevice));
+            BCHK(b, hipMemcpy(b->nloc, to_tile_major(hnloc, NBR_ROWS, ld).data(), sizeof(int) * hnloc.size(),
+                              hipMemcpyHostToDevice));
             BCHK(b, hipMemcpy(b->hoff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice));
             if (!hids.empty())
                 BCHK(b, hipMemcpy(b->hids, hids.data(), sizeof(int) * hids.size(), hipMemcpyHostToDevice));
@@ -326,7 +331,13 @@ int rkdg_batch_set_state(rkdg_batch* b, int32_t i, int32_t basis, const double* 
     if (!b || i < 0 || i >= b->n || !U) return bfail(b, RKDG_ERR_ARG, "batch_set_state");
     BCHK(b, hipSetDevice(b->dev));
     const int E = b->E[i], Np = b->Np, P = b->P;
-    std::vector<double> h((size_t)4 * Np * E);
+    // the instance's layout tiles are one contiguous block of the tile-major state; its
+    // padding columns keep their rest state, so the block is read, updated and written back
+    const size_t blk = (size_t)(E + LAYOUT_TILE - 1) / LAYOUT_TILE * LAYOUT_TILE * 4 * Np;
+    double* dev = b->U[0] + (size_t)b->off[i] * 4 * Np;
+    std::vector<double> h(blk);
+    BCHK(b, hipMemcpyAsync(h.data(), dev, sizeof(double) * blk, hipMemcpyDeviceToHost, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
     parallel_for(E, [&](int a0, int a1) {
         double in[4 * MAXNP], o[4 * MAXNP];
         for (int p = a0; p < a1; ++p) {
@@ -335,12 +346,10 @@ int rkdg_batch_set_state(rkdg_batch* b, int32_t i, int32_t basis, const double* 
                 for (int k = 0; k < Np; ++k) in[v * Np + k] = U[((size_t)v * Np + k) * E + e];
             if (basis == RKDG_BASIS_ELEMENT) element_to_native(P, &b->ex[i][3 * e], &b->ey[i][3 * e], in, o);
             else std::memcpy(o, in, sizeof(double) * 4 * Np);
-            for (int r = 0; r < 4 * Np; ++r) h[(size_t)r * E + p] = o[r];
+            for (int r = 0; r < 4 * Np; ++r) h[tile_index(r, 4 * Np, p)] = o[r];
         }
     });
-    // rows of E doubles into rows of ld doubles, starting at the instance offset
-    BCHK(b, hipMemcpy2DAsync(b->U[0] + b->off[i], sizeof(double) * b->ld, h.data(), sizeof(double) * E,
-                             sizeof(double) * E, 4 * Np, hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipMemcpyAsync(dev, h.data(), sizeof(double) * blk, hipMemcpyHostToDevice, b->stream));
     int rc = bwavespeed_all(b);
     if (rc) return rc;
     BCHK(b, hipStreamSynchronize(b->stream));
@@ -351,15 +360,16 @@ int rkdg_batch_get_state(rkdg_batch* b, int32_t i, int32_t basis, double* U) {
     if (!b || i < 0 || i >= b->n || !U) return bfail(b, RKDG_ERR_ARG, "batch_get_state");
     BCHK(b, hipSetDevice(b->dev));
     const int E = b->E[i], Np = b->Np, P = b->P;
-    std::vector<double> h((size_t)4 * Np * E);
-    BCHK(b, hipMemcpy2DAsync(h.data(), sizeof(double) * E, b->U[0] + b->off[i], sizeof(double) * b->ld,
-                             sizeof(double) * E, 4 * Np, hipMemcpyDeviceToHost, b->stream));
+    const size_t blk = (size_t)(E + LAYOUT_TILE - 1) / LAYOUT_TILE * LAYOUT_TILE * 4 * Np;
+    std::vector<double> h(blk);
+    BCHK(b, hipMemcpyAsync(h.data(), b->U[0] + (size_t)b->off[i] * 4 * Np, sizeof(double) * blk, hipMemcpyDeviceToHost,
+                           b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
     parallel_for(E, [&](int a0, int a1) {
         double in[4 * MAXNP], o[4 * MAXNP];
         for (int p = a0; p < a1; ++p) {
             const int e = b->order[i][p];
-            for (int r = 0; r < 4 * Np; ++r) in[r] = h[(size_t)r * E + p];
+            for (int r = 0; r < 4 * Np; ++r) in[r] = h[tile_index(r, 4 * Np, p)];
             if (basis == RKDG_BASIS_ELEMENT) native_to_element(P, &b->ex[i][3 * e], &b->ey[i][3 * e], in, o);
             else std::memcpy(o, in, sizeof(double) * 4 * Np);
             for (int v = 0; v < 4; ++v)

@@ -450,15 +450,19 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     c->bytes = 3 * ubytes + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size() + sizeof(int) * ld;
     for (int b = 0; b < 3; ++b) HIPCHK(c, hipMemsetAsync(c->U[b], 0, ubytes, c->stream));
     HIPCHK(c, hipMemsetAsync(c->flags, 0, sizeof(int) * ld, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->geo, hgeo.data(), sizeof(double) * hgeo.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->nbr, hnbr.data(), sizeof(int) * hnbr.size(), hipMemcpyHostToDevice, c->stream));
+    // host images are [rows][ld]; the device layout is tile-major
+    const std::vector<double> tgeo = to_tile_major(hgeo, GEO_ROWS, ld);
+    const std::vector<int> tnbr = to_tile_major(hnbr, NBR_ROWS, ld);
+    HIPCHK(c, hipMemcpyAsync(c->geo, tgeo.data(), sizeof(double) * tgeo.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->nbr, tnbr.data(), sizeof(int) * tnbr.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice, c->stream));
     if (c->staged) {
         HIPCHK(c, hipMalloc(&c->nloc, sizeof(int) * hnloc.size()));
         HIPCHK(c, hipMalloc(&c->hoff, sizeof(int) * hoff.size()));
         HIPCHK(c, hipMalloc(&c->hids, sizeof(int) * std::max<size_t>(1, hids.size())));
         c->bytes += sizeof(int) * (hnloc.size() + hoff.size() + hids.size());
-        HIPCHK(c, hipMemcpyAsync(c->nloc, hnloc.data(), sizeof(int) * hnloc.size(), hipMemcpyHostToDevice, c->stream));
+        const std::vector<int> tnloc = to_tile_major(hnloc, NBR_ROWS, ld);
+        HIPCHK(c, hipMemcpyAsync(c->nloc, tnloc.data(), sizeof(int) * tnloc.size(), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->hoff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice, c->stream));
         if (!hids.empty())
             HIPCHK(c, hipMemcpyAsync(c->hids, hids.data(), sizeof(int) * hids.size(), hipMemcpyHostToDevice, c->stream));
@@ -467,7 +471,8 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
         HIPCHK(c, hipMalloc(&c->lim, sizeof(double) * hlim.size()));
         HIPCHK(c, hipMalloc(&c->pairs, sizeof(int) * hpairs.size()));
         c->bytes += sizeof(double) * hlim.size() + sizeof(int) * hpairs.size();
-        HIPCHK(c, hipMemcpyAsync(c->lim, hlim.data(), sizeof(double) * hlim.size(), hipMemcpyHostToDevice, c->stream));
+        const std::vector<double> tlim = to_tile_major(hlim, LIM_ROWS, ld);
+        HIPCHK(c, hipMemcpyAsync(c->lim, tlim.data(), sizeof(double) * tlim.size(), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->pairs, hpairs.data(), sizeof(int) * hpairs.size(), hipMemcpyHostToDevice, c->stream));
     }
     int rc = upload_tables(c->P, ref_basis(c->P).tab);
@@ -507,7 +512,7 @@ int rkdg_set_state(rkdg_ctx* c, int32_t basis, const double* U) {
                 for (int i = 0; i < Np; ++i) in[v * Np + i] = U[((size_t)v * Np + i) * E + e];
             if (basis == RKDG_BASIS_ELEMENT) element_to_native(P, &c->ex[3 * e], &c->ey[3 * e], in, outv);
             else std::memcpy(outv, in, sizeof(double) * 4 * Np);
-            for (int r = 0; r < 4 * Np; ++r) h[(size_t)r * ld + p] = outv[r];
+            for (int r = 0; r < 4 * Np; ++r) h[tile_index(r, 4 * Np, p)] = outv[r];
         }
     });
     HIPCHK(c, hipMemcpyAsync(c->U[0], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
@@ -533,7 +538,7 @@ int rkdg_get_state(rkdg_ctx* c, int32_t basis, double* U) {
         double in[4 * MAXNP], outv[4 * MAXNP];
         for (int p = a; p < b; ++p) {
             int e = c->order[p];
-            for (int r = 0; r < 4 * Np; ++r) in[r] = h[(size_t)r * ld + p];
+            for (int r = 0; r < 4 * Np; ++r) in[r] = h[tile_index(r, 4 * Np, p)];
             if (basis == RKDG_BASIS_ELEMENT) native_to_element(P, &c->ex[3 * e], &c->ey[3 * e], in, outv);
             else std::memcpy(outv, in, sizeof(double) * 4 * Np);
             for (int v = 0; v < 4; ++v)

@@ -4,7 +4,8 @@
 // (calculateConVars, boundaryCondition, calculateVolumeRHS, calculateLFCoeff,
 // calculateEdgeFG, calculateFlux x2, calculateEdgeRHS, rkdgStep*;
 // src/kernelfunctions.cu:84-748).  One lane owns one element:
-//   * its modal state is read once, coalesced, from the SoA layout U[var][mode][ld];
+//   * its modal state is read once, coalesced, from the tile-major layout
+//     U[ld / 256][var][mode][256] (a 256-element tile's modes are one contiguous block);
 //   * the volume integral uses reference-element operators held in constant memory
 //     (uniform across the wavefront, so they are scalar loads) and the element's
 //     inverse Jacobian;
@@ -35,15 +36,15 @@ struct Deg {
     static constexpr int Nqe = P <= 2 ? 3 : 4;
 };
 
-// Element e of row r of an SoA array [rows][ld], addressed as the array base (scalar
-// registers) plus one 32-bit byte offset (every array is below 4 GiB, checked when a
-// context is created): the addressing form of global_load/store with an SGPR base, so no
-// 64-bit address is formed per row.
+// Element e of row r of an array of R rows in the tile-major layout [ld / 256][R][256]
+// (LAYOUT_TILE), addressed as the array base (scalar registers) plus one 32-bit byte
+// offset (every array is below 4 GiB, checked when a context is created): the addressing
+// form of global_load/store with an SGPR base, so no 64-bit address is formed per row.
 template <typename T>
-__device__ __forceinline__ T& soa(T* base, int r, long long ld, int e) {
-    const unsigned off = (unsigned)r * (unsigned)(ld * (long long)sizeof(T)) + (unsigned)e * (unsigned)sizeof(T);
+__device__ __forceinline__ T& tmx(T* base, int r, int R, int e) {
+    const unsigned idx = ((unsigned)(e >> 8) * (unsigned)R + (unsigned)r) * 256u + (unsigned)(e & 255);
     auto b = reinterpret_cast<typename std::conditional<std::is_const<T>::value, const char*, char*>::type>(base);
-    return *reinterpret_cast<T*>(b + off);
+    return *reinterpret_cast<T*>(b + idx * (unsigned)sizeof(T));
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -221,11 +222,11 @@ struct Prefetch {
 };
 
 template <int Np>
-__device__ __forceinline__ void load_modes(const double* __restrict__ U, long long ld, int n, double (&c)[4][Np]) {
+__device__ __forceinline__ void load_modes(const double* __restrict__ U, int n, double (&c)[4][Np]) {
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-        for (int i = 0; i < Np; ++i) c[v][i] = soa(U, v * Np + i, ld, n);
+        for (int i = 0; i < Np; ++i) c[v][i] = tmx(U, v * Np + i, 4 * Np, n);
 }
 
 // Geometry an element needs before its first edge (the rest is loaded one edge ahead).
@@ -278,7 +279,6 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
     constexpr int Np = D::Np, Nqv = D::Nqv, Nqe = D::Nqe;
     constexpr bool PF = NBSRC == NB_GLOBAL && Prefetch<P>::on;
     const DevTables& T = c_tab[P - 1];
-    const long long ld = a.ld;
     const double g = a.gamma;
     auto& R = W.R;
     auto& Nb = W.Nb;
@@ -286,7 +286,7 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
     for (int v = 0; v < 4; ++v)
 #pragma unroll
         for (int i = 0; i < Np; ++i) R[v][i] = 0.0;
-    if (PF && w0 >= 0) load_modes<Np>(a.U, ld, w0 >> 2, Nb);
+    if (PF && w0 >= 0) load_modes<Np>(a.U, w0 >> 2, Nb);
 
     // ---- volume integral: sum_q w_q A (F . grad psi_i) -----------------------------
     // The point loop stays rolled: each iteration reads one row of the reference
@@ -324,8 +324,8 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
         const int w = k == 0 ? w0 : (k == 1 ? w1 : w2);
         const double sx = sx_n, sy = sy_n;
         if (k < 2) {
-            sx_n = -soa(a.geo, 2 * k, ld, e);
-            sy_n = -soa(a.geo, 2 * k + 1, ld, e);
+            sx_n = -tmx(a.geo, 2 * k, GEO_ROWS, e);
+            sy_n = -tmx(a.geo, 2 * k + 1, GEO_ROWS, e);
         }
         double nx, ny, lk;
         edge_frame(sx, sy, eg.area, nx, ny, lk);
@@ -334,7 +334,7 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
         const double* src = nullptr;
         if (w >= 0) {
             if (NBSRC != NB_GLOBAL) src = sU + (w >> 2);
-            else if (!PF) load_modes<Np>(a.U, ld, w >> 2, Nb);
+            else if (!PF) load_modes<Np>(a.U, w >> 2, Nb);
         }
         constexpr int S = staged_stride(P);
         const bool refl = w == -RKDG_BC_REFLECTION;
@@ -485,9 +485,9 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
         // once this edge no longer needs the registers
         if (k < 2) {
             const int wn = k == 0 ? w1 : w2;
-            if (PF && wn >= 0) load_modes<Np>(a.U, ld, wn >> 2, Nb);
+            if (PF && wn >= 0) load_modes<Np>(a.U, wn >> 2, Nb);
         } else if (PF && a.stage > 0) {
-            load_modes<Np>(a.U0, ld, e, Nb);
+            load_modes<Np>(a.U0, e, Nb);
         }
     }
     W.per = per;
@@ -502,7 +502,6 @@ __device__ __forceinline__ void element_update(const StageArgs& a, int e, const 
     constexpr int Np = Deg<P>::Np;
     constexpr bool PF = NBSRC == NB_GLOBAL && Prefetch<P>::on;
     const DevTables& T = c_tab[P - 1];
-    const long long ld = a.ld;
     const double g = a.gamma;
     auto& R = W.R;
     auto& Nb = W.Nb;
@@ -511,7 +510,7 @@ __device__ __forceinline__ void element_update(const StageArgs& a, int e, const 
     // The step-start modes are all loaded before the first store: U0 and Uout are the same
     // buffer in stage 2, so a load placed after a store could not be moved ahead of it and
     // every mode would cost a full memory round trip.
-    if (!PF && a.stage > 0) load_modes<Np>(a.U0, ld, e, Nb);
+    if (!PF && a.stage > 0) load_modes<Np>(a.U0, e, Nb);
     const double ia = frcp(eg.area);
     double mean[4];
 #pragma unroll
@@ -535,7 +534,7 @@ __device__ __forceinline__ void element_update(const StageArgs& a, int e, const 
                 }
             }
             if (i == 0) mean[v] = out;
-            soa(a.Uout, v * Np + i, ld, e) = out;
+            tmx(a.Uout, v * Np + i, 4 * Np, e) = out;
         }
 
     if (a.stage == 2) {
@@ -590,15 +589,14 @@ struct ElemOps {
 };
 
 __device__ __forceinline__ void load_element_ops(const StageArgs& a, const int* nb_ids, int e, ElemOps& o) {
-    const long long ld = a.ld;
-    o.w0 = soa(nb_ids, 0, ld, e);
-    o.w1 = soa(nb_ids, 1, ld, e);
-    o.w2 = soa(nb_ids, 2, ld, e);
-    o.eg.area = soa(a.geo, 4, ld, e);
-    o.eg.xix = soa(a.geo, 0, ld, e);
-    o.eg.xiy = soa(a.geo, 1, ld, e);
-    o.eg.etx = soa(a.geo, 2, ld, e);
-    o.eg.ety = soa(a.geo, 3, ld, e);
+    o.w0 = tmx(nb_ids, 0, NBR_ROWS, e);
+    o.w1 = tmx(nb_ids, 1, NBR_ROWS, e);
+    o.w2 = tmx(nb_ids, 2, NBR_ROWS, e);
+    o.eg.area = tmx(a.geo, 4, GEO_ROWS, e);
+    o.eg.xix = tmx(a.geo, 0, GEO_ROWS, e);
+    o.eg.xiy = tmx(a.geo, 1, GEO_ROWS, e);
+    o.eg.etx = tmx(a.geo, 2, GEO_ROWS, e);
+    o.eg.ety = tmx(a.geo, 3, GEO_ROWS, e);
     o.dtf = a.stage == 2 && a.dtflag[e] != 0;
 }
 
@@ -631,12 +629,13 @@ __device__ __forceinline__ void staged_fill(const StageArgs& a, int tile, int nh
                                             double* sU) {
     using L = Staged<P>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long long ld = a.ld;
     char* lds = (char*)sU;
+    // the tile's rows inside its layout tile: row r starts 2 KiB after row r - 1
     const long long e0 = (long long)tile * L::B;
+    const double* own = a.U + ((e0 >> 8) * L::NR) * 256 + (e0 & 255);
     for (int c = wave; c < L::NR * L::PARTS; c += L::NW) {
         const int r = c / L::PARTS, part = c - r * L::PARTS;
-        const char* src = (const char*)(a.U + r * ld + e0) + part * 1024 + lane * 16;
+        const char* src = (const char*)(own + r * 256) + part * 1024 + lane * 16;
         __builtin_amdgcn_global_load_lds((const void*)src,
                                          (__attribute__((address_space(3))) void*)(lds + r * L::S * 8 + part * 1024), 16, 0, 0);
     }
@@ -644,10 +643,11 @@ __device__ __forceinline__ void staged_fill(const StageArgs& a, int tile, int nh
     for (int hb = 0; hb < L::HCH; ++hb) {
         if (hb * 32 >= nh) break;  // uniform across the workgroup
         const bool on = hb * 32 + (lane >> 1) < nh;
-        const char* src0 = (const char*)a.U + (size_t)hid[hb] * 8 + (lane & 1) * 4;
+        const int h = hid[hb];
+        const char* src0 = (const char*)(a.U + ((size_t)(h >> 8) * L::NR) * 256 + (h & 255)) + (lane & 1) * 4;
         for (int r = wave; r < L::NR; r += L::NW) {
             if (on)
-                __builtin_amdgcn_global_load_lds((const void*)(src0 + (size_t)r * ld * 8),
+                __builtin_amdgcn_global_load_lds((const void*)(src0 + (size_t)r * 2048),
                                                  (__attribute__((address_space(3))) void*)(lds + (r * L::S + L::B + hb * 32) * 8),
                                                  4, 0, 0);
         }
@@ -664,7 +664,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
     stage_bookkeeping(a, ti, dt, done);
     if (done) return;
 
-    const long long ld = a.ld;
     const int e = ti.tile * blockDim.x + threadIdx.x;
     double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
     double Uo[4][Np];
@@ -684,7 +683,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int i = 0; i < Np; ++i) Uo[v][i] = soa(a.U, v * Np + i, ld, e);
+            for (int i = 0; i < Np; ++i) Uo[v][i] = tmx(a.U, v * Np + i, 4 * Np, e);
         load_element_ops(a, a.nbr, e, o);
     }
 
@@ -795,7 +794,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED
 template <int P>
 __global__ void __launch_bounds__(256) k_wavespeed(StageArgs a) {
     constexpr int Np = Deg<P>::Np;
-    const long long ld = a.ld;
     const int nit = iter_count(a);
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         TileInfo ti;
@@ -804,12 +802,13 @@ __global__ void __launch_bounds__(256) k_wavespeed(StageArgs a) {
         double s_l = 0.0;
         if (e < a.E) {
             const double g = a.gamma;
-            const double rho = a.U[(0 * Np) * ld + e];
-            const double u = a.U[(1 * Np) * ld + e] / rho, v = a.U[(2 * Np) * ld + e] / rho;
-            const double p = (g - 1.0) * (a.U[(3 * Np) * ld + e] - 0.5 * rho * (u * u + v * v));
+            const double rho = tmx(a.U, 0 * Np, 4 * Np, e);
+            const double u = tmx(a.U, 1 * Np, 4 * Np, e) / rho, v = tmx(a.U, 2 * Np, 4 * Np, e) / rho;
+            const double p = (g - 1.0) * (tmx(a.U, 3 * Np, 4 * Np, e) - 0.5 * rho * (u * u + v * v));
             if (!(rho > 0.0) || !(p > 0.0)) atomicMin(&ti.st->err_pos, e);
             const double c = sqrt(g * p / rho);
-            const ElemGeo eg{a.geo[0 * ld + e], a.geo[1 * ld + e], a.geo[2 * ld + e], a.geo[3 * ld + e], a.geo[4 * ld + e]};
+            const ElemGeo eg{tmx(a.geo, 0, GEO_ROWS, e), tmx(a.geo, 1, GEO_ROWS, e), tmx(a.geo, 2, GEO_ROWS, e),
+                             tmx(a.geo, 3, GEO_ROWS, e), tmx(a.geo, 4, GEO_ROWS, e)};
             double per = 0.0;
             for (int k = 0; k < 3; ++k) {
                 double nx, ny, lk;
@@ -849,19 +848,18 @@ __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
     const DevTables& T = c_tab[P - 1];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
-    const long long ld = a.ld;
     double c[4][Np], nm[3][4];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-        for (int i = 0; i < Np; ++i) c[v][i] = a.U[(v * Np + i) * ld + e];
+        for (int i = 0; i < Np; ++i) c[v][i] = tmx(a.U, v * Np + i, 4 * Np, e);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const int w = a.nbr[k * ld + e];
+        const int w = tmx(a.nbr, k, NBR_ROWS, e);
         if (w >= 0) {
             const int n = w >> 2;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) nm[k][v] = a.U[(v * Np) * ld + n];
+            for (int v = 0; v < 4; ++v) nm[k][v] = tmx(a.U, v * Np, 4 * Np, n);
         } else {
 #pragma unroll
             for (int v = 0; v < 4; ++v) nm[k][v] = c[v][0];
@@ -869,8 +867,8 @@ __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
     }
     double al[6];
 #pragma unroll
-    for (int r = 0; r < 6; ++r) al[r] = a.lim[r * ld + e];
-    const double tol = a.M * a.lim[6 * ld + e];
+    for (int r = 0; r < 6; ++r) al[r] = tmx(a.lim, r, LIM_ROWS, e);
+    const double tol = a.M * tmx(a.lim, 6, LIM_ROWS, e);
     const int pk = a.pairs[e];
     const double nu = 1.5;
     int flag = 0;
@@ -955,7 +953,7 @@ __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int i = 1; i < Np; ++i) a.U[(v * Np + i) * ld + e] = c[v][i];
+            for (int i = 1; i < Np; ++i) tmx(a.U, v * Np + i, 4 * Np, e) = c[v][i];
     }
     a.flags[e] = flag;
 }
