@@ -272,11 +272,12 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     c->prm = *prm;
     c->P = prm->degree; c->Np = np_of(c->P); c->Nqv = nqv_of(c->P); c->Nqe = nqe_of(c->P);
     c->E = m->n_elements;
-    c->block = prm->block > 0 ? prm->block : env_int("RKDG_BLOCK", prm->degree >= 3 ? 128 : 256);
-    // precision form: staged tiles of 256 elements at P1/P2 (lane per element), else 64
-    // elements on 256 threads (four lanes per element)
-    const bool px = prm->precision == RKDG_PREC_FP32_MFMA && c->P <= 2 && env_int("RKDG_PX", 1);
-    if (prm->precision == RKDG_PREC_FP32_MFMA) c->block = px ? staged_block(c->P) : 64;
+    // P3: 64-element tiles (measured 3.5% faster than 128 and 10% faster than 256 at 2.1M)
+    c->block = prm->block > 0 ? prm->block : env_int("RKDG_BLOCK", prm->degree >= 3 ? 64 : 256);
+    // precision form: staged tiles of 256 elements (lane per element), else 64 elements on
+    // 256 threads (four lanes per element)
+    const bool px = prm->precision == RKDG_PREC_FP32_MFMA && env_int("RKDG_PX", 1);
+    if (prm->precision == RKDG_PREC_FP32_MFMA) c->block = px ? px_block(c->P) : 64;
     if (c->block % 64 || c->block < 64 || c->block > 256) { delete c; return fail(nullptr, RKDG_ERR_ARG, "block must be 64..256, multiple of 64"); }
     c->xcd_remap = env_int("RKDG_XCD_REMAP", 1);
     c->ntiles = (c->E + c->block - 1) / c->block;
@@ -409,13 +410,14 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     // staged form: per-tile halos, used when the tile's LDS image fits
     std::vector<int> hoff, hids, hnloc;
     // (measured faster than the global-gather form at P1 and P2)
-    if ((prm->precision == RKDG_PREC_FP64 || px) && c->block == staged_block(c->P) &&
-        env_int("RKDG_STAGED", c->P <= 2 ? 1 : 0)) {
+    if (px || (prm->precision == RKDG_PREC_FP64 && c->block == staged_block(c->P) &&
+               env_int("RKDG_STAGED", c->P <= 2 ? 1 : 0))) {
         const int hmax = build_halo(hnbr.data(), ld, E, c->block, hoff, hids, hnloc);
-        const long long lds = (long long)4 * c->Np * staged_stride(c->P) * (long long)sizeof(double);
-        if (c->block + hmax <= staged_stride(c->P) && lds <= max_lds_per_block(prm->device)) {
+        const int stride = px ? px_stride(c->P) : staged_stride(c->P);
+        const long long lds = px ? px_lds_bytes(c->P) : (long long)4 * c->Np * stride * (long long)sizeof(double);
+        if (c->block + hmax <= stride && lds <= max_lds_per_block(prm->device)) {
             c->staged = 1;
-            c->lds_stride = staged_stride(c->P);
+            c->lds_stride = stride;
             int cus = 0;
             c->pipe = env_int("RKDG_PIPE", 0);
             const int per_cu = env_int("RKDG_PERSIST", c->pipe ? 2 : 0);

@@ -197,6 +197,14 @@ struct LimArgs {
 // 64 KiB: 4 Np rows of stride doubles).
 constexpr int staged_block(int P) { return P <= 2 ? 256 : 128; }
 constexpr int staged_stride(int P) { return P <= 2 ? 352 : 200; }
+// Precision form: 256-element staged tiles at every degree; at P3 its LDS image holds FP32
+// modes and FP64 cell means (k_stage_pz), at P1/P2 the FP64 image of the FP64 form.
+constexpr int px_block(int P) { return 256; }
+constexpr int px_stride(int P) { return 352; }
+constexpr long long px_lds_bytes(int P) {
+    return P <= 2 ? 4LL * ((P + 1) * (P + 2) / 2) * px_stride(P) * 8
+                  : 4LL * ((P + 1) * (P + 2) / 2) * px_stride(P) * 4 + 4LL * px_stride(P) * 8;
+}
 
 int upload_tables(int P, const DevTables& t);  // per current device
 int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream);

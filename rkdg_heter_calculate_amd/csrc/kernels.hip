@@ -47,6 +47,27 @@ __device__ __forceinline__ T& tmx(T* base, int r, int R, int e) {
     return *reinterpret_cast<T*>(b + idx * (unsigned)sizeof(T));
 }
 
+// The same arrays as buffer resources: one VGPR byte offset per element (tm_voff) and the
+// row as a scalar byte offset (row r of a layout tile starts r * 2048 bytes after row 0), so
+// a loop over rows costs no per-row address registers.  Raw buffer accesses are range-checked
+// against the array size: reads outside return 0 and writes outside are dropped.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tm_rsrc(const void* base, long long ld, int R, int size = 8) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(unsigned)(ld * R * size), 0x00020000);
+}
+__device__ __forceinline__ unsigned tm_voff(int e, int R, int size = 8) {
+    return (((unsigned)(e >> 8) * (unsigned)R) * 256u + (unsigned)(e & 255)) * (unsigned)size;
+}
+__device__ __forceinline__ int tm_load_i32(__amdgpu_buffer_rsrc_t rs, unsigned voff, int r) {
+    return __builtin_bit_cast(int, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, r * 1024, 0));
+}
+__device__ __forceinline__ double tm_load(__amdgpu_buffer_rsrc_t rs, unsigned voff, int r) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, r * 2048, 0));
+}
+typedef unsigned int u32x2 __attribute__((__vector_size__(8)));
+__device__ __forceinline__ void tm_store(__amdgpu_buffer_rsrc_t rs, unsigned voff, int r, double x) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), rs, (int)voff, r * 2048, 0);
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -54,6 +75,16 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // Block-wide max of three non-negative values, then one atomic per value.
+// Raises *p (the bits of a non-negative double, which order like the values) to x.  The
+// atomic is skipped when the current value already covers x: one workgroup per tile would
+// otherwise queue tens of thousands of atomics on one address per stage.  The final value is
+// the maximum either way (values only increase), so the result stays order-independent.
+__device__ __forceinline__ void atomic_raise(unsigned long long* p, double x) {
+    if (!(x > 0)) return;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bits) atomicMax(p, bits);
+}
+
 __device__ __forceinline__ void block_max3_atomic(double a, double b, double c, unsigned long long* pa,
                                                   unsigned long long* pb, unsigned long long* pc, bool do_bc) {
     __shared__ double red[3][16];
@@ -65,10 +96,10 @@ __device__ __forceinline__ void block_max3_atomic(double a, double b, double c, 
     if (threadIdx.x == 0) {
         double ma = 0, mb = 0, mc = 0;
         for (int w = 0; w < nw; ++w) { ma = fmax(ma, red[0][w]); mb = fmax(mb, red[1][w]); mc = fmax(mc, red[2][w]); }
-        if (ma > 0) atomicMax(pa, (unsigned long long)__double_as_longlong(ma));
+        atomic_raise(pa, ma);
         if (do_bc) {
-            if (mb > 0) atomicMax(pb, (unsigned long long)__double_as_longlong(mb));
-            if (mc > 0) atomicMax(pc, (unsigned long long)__double_as_longlong(mc));
+            atomic_raise(pb, mb);
+            atomic_raise(pc, mc);
         }
     }
 }
@@ -789,6 +820,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED
 
 #include "stage_mx.inc"
 #include "stage_px.inc"
+#include "stage_pz.inc"
 
 // CFL wave speed of the current state, max-reduced into ws[slot]
 template <int P>
@@ -1045,15 +1077,29 @@ static void launch_stage_px_p(int flux, const StageArgs& a, hipStream_t s) {
     else launch_stage_px_pf<P, RKDG_FLUX_LF>(a, s);
 }
 
+template <int FLUX>
+static void launch_stage_pz_f(const StageArgs& a, hipStream_t s) {
+    static bool opted = false;  // above 64 KiB the kernel must opt in
+    if (!opted) {
+        hipFuncSetAttribute((const void*)k_stage_pz<3, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Pz<3>::LDS);
+        opted = true;
+    }
+    hipLaunchKernelGGL((k_stage_pz<3, FLUX>), dim3(grid_blocks(a)), dim3(Pz<3>::B), Pz<3>::LDS, s, a);
+}
+
 int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     if (prec == RKDG_PREC_FP32_MFMA) {
         // staged tiles of 256 elements, one lane per element (P1, P2)
         if (a.staged) {
-            if (block != 256) return (int)hipErrorInvalidValue;
+            if (block != px_block(P)) return (int)hipErrorInvalidValue;
             switch (P) {
                 case 1: launch_stage_px_p<1>(flux, a, s); break;
                 case 2: launch_stage_px_p<2>(flux, a, s); break;
+                case 3:
+                    if (flux == RKDG_FLUX_ROE) launch_stage_pz_f<RKDG_FLUX_ROE>(a, s);
+                    else launch_stage_pz_f<RKDG_FLUX_LF>(a, s);
+                    break;
                 default: return (int)hipErrorInvalidValue;
             }
             return (int)hipGetLastError();
