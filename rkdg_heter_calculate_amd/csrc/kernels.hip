@@ -541,7 +541,16 @@ __device__ __forceinline__ void element_update(const StageArgs& a, int e, const 
     // The step-start modes are all loaded before the first store: U0 and Uout are the same
     // buffer in stage 2, so a load placed after a store could not be moved ahead of it and
     // every mode would cost a full memory round trip.
-    if (!PF && a.stage > 0) load_modes<Np>(a.U0, e, Nb);
+    // (buffer-resource addressing: the 4 Np rows cost no per-row address registers)
+    const __amdgpu_buffer_rsrc_t rO = tm_rsrc(a.Uout, a.ld, 4 * Np);
+    const unsigned vo = tm_voff(e, 4 * Np);
+    if (!PF && a.stage > 0) {
+        const __amdgpu_buffer_rsrc_t r0 = tm_rsrc(a.U0, a.ld, 4 * Np);
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 0; i < Np; ++i) Nb[v][i] = tm_load(r0, vo, v * Np + i);
+    }
     const double ia = frcp(eg.area);
     double mean[4];
 #pragma unroll
@@ -565,7 +574,7 @@ __device__ __forceinline__ void element_update(const StageArgs& a, int e, const 
                 }
             }
             if (i == 0) mean[v] = out;
-            tmx(a.Uout, v * Np + i, 4 * Np, e) = out;
+            tm_store(rO, vo, v * Np + i, out);
         }
 
     if (a.stage == 2) {
