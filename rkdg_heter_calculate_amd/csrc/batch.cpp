@@ -1,8 +1,9 @@
 // Batching form: many independent meshes of uneven size advance together.
 //
-// All instances are concatenated into one SoA layout; each instance starts on a
-// 256-element boundary, so a workgroup tile (64, 128 or 256 elements) never spans two
-// instances and carries a single instance id (tile_mesh).  Every instance keeps its own
+// All instances are concatenated into one tile-major layout; each instance starts on a
+// 256-element boundary, so a layout tile and a workgroup tile (64, 128 or 256 elements)
+// never span two instances, an instance's state is one contiguous block, and a workgroup
+// tile carries a single instance id (tile_mesh).  Every instance keeps its own
 // DevState, so its CFL dt is reduced and consumed on the device without any host
 // synchronisation.  One launch per RK stage advances every instance; the stage kernels
 // run persistently (grid capped at a few workgroups per CU, each looping over tiles),

@@ -260,7 +260,8 @@ __device__ __forceinline__ void load_modes(const double* __restrict__ U, int n, 
         for (int i = 0; i < Np; ++i) c[v][i] = tmx(U, v * Np + i, 4 * Np, n);
 }
 
-// Geometry an element needs before its first edge (the rest is loaded one edge ahead).
+// Element geometry: the rows of the inverse Jacobian and the area (edge normals and lengths
+// are derived from them, edge_normal).
 struct ElemGeo {
     double xix, xiy, etx, ety, area;
 };
