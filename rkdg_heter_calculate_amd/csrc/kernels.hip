@@ -150,6 +150,22 @@ __device__ __forceinline__ double pressure(double g, const double* U, double ir)
 // Roe (1981) normal flux with the Harten-Hyman (1983) entropy fix on the acoustic waves.
 // The central part 0.5 (F(UL) + F(UR)).n is formed from the two directional fluxes
 // directly, so only four partial sums stay live instead of both F and G of both states.
+// Square root of a positive normal double: two Goldschmidt steps from the hardware
+// reciprocal-square-root estimate (no scaling for denormals, zero or infinity), for the
+// Roe flux's five square roots per point (densities and sound speeds of physical states):
+// about 6 instructions fewer than the general square root, 2-5% of a P1 Roe stage.
+__device__ __forceinline__ double fsqrt(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double gg = x * y, h = 0.5 * y;
+    const double r = fma(-gg, h, 0.5);
+    gg = fma(gg, r, gg);
+    h = fma(h, r, h);
+    double d = fma(-gg, gg, x);
+    gg = fma(d, h, gg);
+    d = fma(-gg, gg, x);
+    return fma(d, h, gg);
+}
+
 __device__ __forceinline__ void roe_flux(double g, const double* UL, const double* UR, double nx, double ny,
                                          double* Fh) {
     const double rl = UL[0], irl = frcp(rl), ul = UL[1] * irl, vl = UL[2] * irl;
@@ -162,12 +178,12 @@ __device__ __forceinline__ void roe_flux(double g, const double* UL, const doubl
 #pragma unroll
     for (int v = 0; v < 4; ++v) Fh[v] = FL[v] + FR[v];
     const double hl = (UL[3] + pl) * irl, hr = (UR[3] + pr) * irr;
-    const double sl = sqrt(rl), sr = sqrt(rr), isum = frcp(sl + sr);
+    const double sl = fsqrt(rl), sr = fsqrt(rr), isum = frcp(sl + sr);
     const double rt = sl * sr;
     const double ut = (sl * ul + sr * ur) * isum, vt = (sl * vl + sr * vr) * isum, ht = (sl * hl + sr * hr) * isum;
     const double q2 = ut * ut + vt * vt;
     const double c2 = (g - 1) * (ht - 0.5 * q2);
-    const double ct = sqrt(c2), ic2 = frcp(c2);
+    const double ct = fsqrt(c2), ic2 = frcp(c2);
     const double unt = ut * nx + vt * ny, utt = -ut * ny + vt * nx;
     const double dp = pr - pl, dr = rr - rl;
     const double unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
@@ -178,7 +194,7 @@ __device__ __forceinline__ void roe_flux(double g, const double* UL, const doubl
     const double a3 = rt * dut;
     const double a4 = 0.5 * (dp + rt * ct * dun) * ic2;
     // Harten-Hyman entropy fix on the acoustic waves (vanishes across shocks)
-    const double cl = sqrt(g * pl * irl), cr = sqrt(g * pr * irr);
+    const double cl = fsqrt(g * pl * irl), cr = fsqrt(g * pr * irr);
     double l1 = fabs(unt - ct), l4 = fabs(unt + ct);
     const double l2 = fabs(unt);
     const double d1 = fmax(0.0, fmax((unt - ct) - (unl - cl), (unr - cr) - (unt - ct)));
