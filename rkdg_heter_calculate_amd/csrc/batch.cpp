@@ -300,6 +300,12 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
         b->block = staged_block(b->P);
         b->staged = 1;
     }
+    // FP64: the global-gather tiles until rkdg_batch_autotune picks; RKDG_BATCH_STAGED=1
+    // starts from the staged tiles instead (when every halo fits)
+    if (p0.precision == RKDG_PREC_FP64 && b->staged_ok && env_int("RKDG_BATCH_STAGED", 0)) {
+        b->block = staged_block(b->P);
+        b->staged = 1;
+    }
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;

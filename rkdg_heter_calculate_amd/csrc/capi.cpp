@@ -135,11 +135,6 @@ static int fail(rkdg_ctx* c, int code, const std::string& msg) {
         if (_e != hipSuccess) return fail(ctx, RKDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
-static int env_int(const char* name, int dflt) {
-    const char* s = std::getenv(name);
-    return s ? std::atoi(s) : dflt;
-}
-
 static StageArgs make_args(rkdg_ctx* c, int stage, int slot, int dt_mode, double dt, double t_end) {
     StageArgs a{};
     const int u_in = stage == 0 ? 0 : stage;     // stage s reads buffer s (0, 1, 2)

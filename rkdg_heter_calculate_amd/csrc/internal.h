@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdlib>
 #include <functional>
 #include <string>
 #include <vector>
@@ -27,6 +28,12 @@ constexpr int NBR_ROWS = 3;   // neighbour codes (and LDS slots) across edges 0,
 // ld elements apart.  Measured on MI355X: 7.2 TB/s for reading such blocks versus 5.6 TB/s
 // for the same bytes as rows of an [R][ld] array; 5.6 versus 4.7 TB/s read plus write.
 constexpr int LAYOUT_TILE = 256;
+
+// Integer tuning switch from the environment (RKDG_*), or dflt when unset.
+inline int env_int(const char* name, int dflt) {
+    const char* s = std::getenv(name);
+    return s ? std::atoi(s) : dflt;
+}
 inline size_t tile_index(int r, int R, long long p) {
     return ((size_t)(p / LAYOUT_TILE) * R + r) * LAYOUT_TILE + (size_t)(p % LAYOUT_TILE);
 }

@@ -147,8 +147,11 @@ def test_advance_counts_steps_per_call():
     s.close()
 
 
-def test_batch_equals_single():
-    print("case: batch vs single", flush=True)
+@pytest.mark.parametrize("P,staged", [(2, False), (2, True), (1, True)])
+def test_batch_equals_single(P, staged, monkeypatch):
+    print(f"case: batch vs single P{P} staged={staged}", flush=True)
+    if staged:
+        monkeypatch.setenv("RKDG_BATCH_STAGED", "1")  # the batch's staged tiles from the start
     import ctypes
     from rkdg_heter_calculate_amd import _lib as L
     from rkdg_heter_calculate_amd.solver import make_params, mesh_view
@@ -160,7 +163,7 @@ def test_batch_equals_single():
         meshes[i] = v
     params = (L.Params * 3)()
     for i in range(3):
-        params[i] = make_params(degree=2, dt_policy="max", cfl=0.25 + 0.02 * i)
+        params[i] = make_params(degree=P, dt_policy="max", cfl=0.25 + 0.02 * i)
     h = ctypes.c_void_p()
     L.check(lib.rkdg_batch_create(3, meshes, params, ctypes.byref(h)), None, "batch_create")
     vp = np.array(vortex_params(), dtype=np.float64)
@@ -169,10 +172,11 @@ def test_batch_equals_single():
     L.check(lib.rkdg_batch_advance(h, 9, 1), None, "advance")
     L.check(lib.rkdg_batch_sync(h), None, "sync")
     for i, gr in enumerate(grids):
-        s = Solver(gr, degree=2, dt_policy="max", cfl=0.25 + 0.02 * i)
+        s = Solver(gr, degree=P, dt_policy="max", cfl=0.25 + 0.02 * i)
         s.project_ic(1, vortex_params())
         s.advance(9)
-        Ub = np.empty((4, 6, gr.n_elements))
+        Np = (P + 1) * (P + 2) // 2
+        Ub = np.empty((4, Np, gr.n_elements))
         L.check(lib.rkdg_batch_get_state(h, i, 1, L.dptr(Ub)), None, "get")
         # both are FP64 forms of the same scheme; they may fuse multiply-adds differently
         Us = s.get_state(basis=1)
