@@ -81,17 +81,17 @@ def batch_point(count, steps, precision="fp64"):
         meshes[i] = views[i]
         params[i] = make_params(degree=2, flux="lf", dt_policy="max", cfl=inst[i][2], precision=precision)
     h = ctypes.c_void_p()
-    L.check(lib.rkdg_batch_create(count, meshes, params, ctypes.byref(h)), None, "batch_create")
+    L.bcheck(lib.rkdg_batch_create(count, meshes, params, ctypes.byref(h)), None, "batch_create")
     for i in range(count):
         vp = np.array(inst[i][1], dtype=np.float64)
-        L.check(lib.rkdg_batch_project_ic(h, i, 1, L.dptr(vp), len(vp)), None, "batch_project_ic")
+        L.bcheck(lib.rkdg_batch_project_ic(h, i, 1, L.dptr(vp), len(vp)), h, "batch_project_ic")
     best = ctypes.c_int32()
-    L.check(lib.rkdg_batch_autotune(h, ctypes.byref(best)), None, "autotune")
-    L.check(lib.rkdg_batch_advance(h, 2, 1), None, "warmup")
-    L.check(lib.rkdg_batch_sync(h), None, "sync")
+    L.bcheck(lib.rkdg_batch_autotune(h, ctypes.byref(best)), h, "autotune")
+    L.bcheck(lib.rkdg_batch_advance(h, 2, 1), h, "warmup")
+    L.bcheck(lib.rkdg_batch_sync(h), h, "sync")
     t0 = time.perf_counter()
-    L.check(lib.rkdg_batch_advance(h, steps, 1), None, "advance")
-    L.check(lib.rkdg_batch_sync(h), None, "sync")
+    L.bcheck(lib.rkdg_batch_advance(h, steps, 1), h, "advance")
+    L.bcheck(lib.rkdg_batch_sync(h), h, "sync")
     dt = time.perf_counter() - t0
     info = np.zeros(4, np.int64)
     lib.rkdg_batch_info(h, info.ctypes.data_as(L.P_int64))

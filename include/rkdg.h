@@ -189,6 +189,7 @@ int rkdg_batch_sync(rkdg_batch* b);
  * best_block may be NULL. */
 int rkdg_batch_autotune(rkdg_batch* b, int32_t* best_block);
 int rkdg_batch_info(rkdg_batch* b, int64_t out[4]);   /* instances, total elements, block, device bytes */
+const char* rkdg_batch_last_error(rkdg_batch* b);     /* NULL b: last batch error of this thread */
 
 /* ---- host mesh utilities (no GPU) ------------------------------------------------ */
 /* Owning mesh container filled by the library. */

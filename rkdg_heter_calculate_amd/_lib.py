@@ -87,6 +87,7 @@ SIGNATURES = [
     ("rkdg_batch_sync", c_int32, [ctypes.c_void_p]),
     ("rkdg_batch_autotune", c_int32, [ctypes.c_void_p, P_int32]),
     ("rkdg_batch_info", c_int32, [ctypes.c_void_p, P_int64]),
+    ("rkdg_batch_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
     ("rkdg_grid_splitquad", c_int32, [c_int32, c_int32, c_double, c_double, c_double, c_double, c_int32, c_int32,
                                       c_double, ctypes.c_uint64, c_int32, ctypes.POINTER(Grid)]),
     ("rkdg_grid_read", c_int32, [ctypes.c_char_p, ctypes.c_char_p, c_double, ctypes.POINTER(Grid)]),
@@ -130,6 +131,16 @@ def check(rc, ctx=None, what=""):
     if rc != OK:
         lib = load()
         msg = lib.rkdg_last_error(ctx)
+        msg = msg.decode() if msg else ""
+        raise RkdgError(rc, f"{what}: {msg}")
+    return rc
+
+
+def bcheck(rc, batch=None, what=""):
+    """check() for the rkdg_batch_* calls, whose messages live on the batch."""
+    if rc != OK:
+        lib = load()
+        msg = lib.rkdg_batch_last_error(batch)
         msg = msg.decode() if msg else ""
         raise RkdgError(rc, f"{what}: {msg}")
     return rc

@@ -64,6 +64,17 @@ static int bfail(rkdg_batch* b, int code, const std::string& m) {
         if (_e != hipSuccess) return bfail(b, RKDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
+// set-up: a failed HIP call releases the half-built batch before it reports
+#define BCREATECHK(call)                                                                       \
+    do {                                                                                       \
+        hipError_t _e = (call);                                                                \
+        if (_e != hipSuccess) {                                                                \
+            std::string _m = std::string(#call) + ": " + hipGetErrorString(_e);                \
+            rkdg_batch_destroy(b);                                                             \
+            return bfail(nullptr, _e == hipErrorOutOfMemory ? RKDG_ERR_NOMEM : RKDG_ERR_HIP, _m); \
+        }                                                                                      \
+    } while (0)
+
 static int upload_tiles(rkdg_batch* b) {
     b->ntiles = (int)(b->ld / b->block);
     std::vector<int> tm(b->ntiles);
@@ -232,9 +243,23 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
             for (int k = 0; k < 3; ++k) {
                 int nb = m.neighbour[3 * e + k];
                 if (nb >= E) { merr = "neighbour id out of range"; break; }
-                if (nb >= 0) hnbr[(size_t)k * ld + off + p] = ((off + pos[nb]) << 2) | se[3 * e + k];
-                else hnbr[(size_t)k * ld + off + p] = -(m.boundary ? m.boundary[3 * e + k] : RKDG_BC_OUTFLOW);
+                if (nb >= 0) {
+                    const int sk = se[3 * e + k];
+                    if (sk < 0 || sk > 2 || m.neighbour[3 * nb + sk] != e) {
+                        merr = "neighbour relation not symmetric at element " + std::to_string(e);
+                        break;
+                    }
+                    hnbr[(size_t)k * ld + off + p] = ((off + pos[nb]) << 2) | sk;
+                } else {
+                    const int t = m.boundary ? m.boundary[3 * e + k] : RKDG_BC_OUTFLOW;
+                    if (t != RKDG_BC_FARFIELD && t != RKDG_BC_OUTFLOW && t != RKDG_BC_REFLECTION) {
+                        merr = "unknown boundary tag";
+                        break;
+                    }
+                    hnbr[(size_t)k * ld + off + p] = -t;
+                }
             }
+            if (!merr.empty()) break;
             int id = m.dt_id ? m.dt_id[e] : e;
             bool inc = true;
             if (p0.dt_policy == RKDG_DT_REFERENCE) {
@@ -249,8 +274,8 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     for (int i = 0; i < n; ++i) hcfl[i] = params[i].cfl;
 
     b->dev = p0.device;
-    BCHK(b, hipSetDevice(b->dev));
-    BCHK(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BCREATECHK(hipSetDevice(b->dev));
+    BCREATECHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     size_t ub = sizeof(double) * hU.size();
     // the kernels address every array with 32-bit byte offsets
     if (ub >= (1ull << 32) || sizeof(double) * hgeo.size() >= (1ull << 32)) {
@@ -261,19 +286,19 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     {
         const std::vector<double> tU = to_tile_major(hU, 4 * b->Np, ld);
         for (int k = 0; k < 3; ++k) {
-            BCHK(b, hipMalloc(&b->U[k], ub));
-            BCHK(b, hipMemcpy(b->U[k], tU.data(), ub, hipMemcpyHostToDevice));
+            BCREATECHK(hipMalloc(&b->U[k], ub));
+            BCREATECHK(hipMemcpy(b->U[k], tU.data(), ub, hipMemcpyHostToDevice));
         }
     }
-    BCHK(b, hipMalloc(&b->geo, sizeof(double) * hgeo.size()));
-    BCHK(b, hipMalloc(&b->nbr, sizeof(int) * hnbr.size()));
-    BCHK(b, hipMalloc(&b->dtflag, hflag.size()));
-    BCHK(b, hipMalloc(&b->st, sizeof(DevState) * n));
-    BCHK(b, hipMalloc(&b->cfl, sizeof(double) * n));
-    BCHK(b, hipMemcpy(b->geo, to_tile_major(hgeo, GEO_ROWS, ld).data(), sizeof(double) * hgeo.size(), hipMemcpyHostToDevice));
-    BCHK(b, hipMemcpy(b->nbr, to_tile_major(hnbr, NBR_ROWS, ld).data(), sizeof(int) * hnbr.size(), hipMemcpyHostToDevice));
-    BCHK(b, hipMemcpy(b->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice));
-    BCHK(b, hipMemcpy(b->cfl, hcfl.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    BCREATECHK(hipMalloc(&b->geo, sizeof(double) * hgeo.size()));
+    BCREATECHK(hipMalloc(&b->nbr, sizeof(int) * hnbr.size()));
+    BCREATECHK(hipMalloc(&b->dtflag, hflag.size()));
+    BCREATECHK(hipMalloc(&b->st, sizeof(DevState) * n));
+    BCREATECHK(hipMalloc(&b->cfl, sizeof(double) * n));
+    BCREATECHK(hipMemcpy(b->geo, to_tile_major(hgeo, GEO_ROWS, ld).data(), sizeof(double) * hgeo.size(), hipMemcpyHostToDevice));
+    BCREATECHK(hipMemcpy(b->nbr, to_tile_major(hnbr, NBR_ROWS, ld).data(), sizeof(int) * hnbr.size(), hipMemcpyHostToDevice));
+    BCREATECHK(hipMemcpy(b->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice));
+    BCREATECHK(hipMemcpy(b->cfl, hcfl.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     b->bytes = 3 * ub + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size();
     if (b->P <= 2) {
         // staged-form tables for 256-element tiles (instances start on 256-element
@@ -282,14 +307,14 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
         const int B = staged_block(b->P);
         const int hmax = build_halo(hnbr.data(), ld, (int)ld, B, hoff, hids, hnloc);
         if (B + hmax <= staged_stride(b->P)) {
-            BCHK(b, hipMalloc(&b->nloc, sizeof(int) * hnloc.size()));
-            BCHK(b, hipMalloc(&b->hoff, sizeof(int) * hoff.size()));
-            BCHK(b, hipMalloc(&b->hids, sizeof(int) * std::max<size_t>(1, hids.size())));
-            BCHK(b, hipMemcpy(b->nloc, to_tile_major(hnloc, NBR_ROWS, ld).data(), sizeof(int) * hnloc.size(),
+            BCREATECHK(hipMalloc(&b->nloc, sizeof(int) * hnloc.size()));
+            BCREATECHK(hipMalloc(&b->hoff, sizeof(int) * hoff.size()));
+            BCREATECHK(hipMalloc(&b->hids, sizeof(int) * std::max<size_t>(1, hids.size())));
+            BCREATECHK(hipMemcpy(b->nloc, to_tile_major(hnloc, NBR_ROWS, ld).data(), sizeof(int) * hnloc.size(),
                               hipMemcpyHostToDevice));
-            BCHK(b, hipMemcpy(b->hoff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice));
+            BCREATECHK(hipMemcpy(b->hoff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice));
             if (!hids.empty())
-                BCHK(b, hipMemcpy(b->hids, hids.data(), sizeof(int) * hids.size(), hipMemcpyHostToDevice));
+                BCREATECHK(hipMemcpy(b->hids, hids.data(), sizeof(int) * hids.size(), hipMemcpyHostToDevice));
             b->bytes += sizeof(int) * (hnloc.size() + hoff.size() + hids.size());
             b->staged_ok = true;
         }
@@ -316,10 +341,12 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     if (rc) { std::string m2 = b->err; rkdg_batch_destroy(b); return bfail(nullptr, rc, m2); }
     rc = bwavespeed_all(b);
     if (rc) { std::string m2 = b->err; rkdg_batch_destroy(b); return bfail(nullptr, rc, m2); }
-    BCHK(b, hipStreamSynchronize(b->stream));
+    BCREATECHK(hipStreamSynchronize(b->stream));
     *out = b;
     return RKDG_OK;
 }
+
+const char* rkdg_batch_last_error(rkdg_batch* b) { return b ? b->err.c_str() : g_berr.c_str(); }
 
 int rkdg_batch_destroy(rkdg_batch* b) {
     if (!b) return RKDG_OK;

@@ -135,6 +135,19 @@ static int fail(rkdg_ctx* c, int code, const std::string& msg) {
         if (_e != hipSuccess) return fail(ctx, RKDG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
+static int hip_code(hipError_t e) { return e == hipErrorOutOfMemory ? RKDG_ERR_NOMEM : RKDG_ERR_HIP; }
+
+// set-up: a failed HIP call releases the half-built context before it reports
+#define CREATECHK(call)                                                                       \
+    do {                                                                                      \
+        hipError_t _e = (call);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            std::string _m = std::string(#call) + ": " + hipGetErrorString(_e);               \
+            rkdg_destroy(c);                                                                  \
+            return fail(nullptr, hip_code(_e), _m);                                           \
+        }                                                                                     \
+    } while (0)
+
 static StageArgs make_args(rkdg_ctx* c, int stage, int slot, int dt_mode, double dt, double t_end) {
     StageArgs a{};
     const int u_in = stage == 0 ? 0 : stage;     // stage s reads buffer s (0, 1, 2)
@@ -325,6 +338,11 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     std::vector<int32_t> se((size_t)3 * E);
     if (m->shared_edge) {
         std::memcpy(se.data(), m->shared_edge, sizeof(int32_t) * 3 * E);
+        for (size_t i = 0; i < se.size(); ++i)
+            if (m->neighbour[i] >= 0 && (se[i] < 0 || se[i] > 2)) {
+                delete c;
+                return fail(nullptr, RKDG_ERR_MESH, "shared edge index outside 0..2 at element " + std::to_string(i / 3));
+            }
     } else {
         std::string err;
         int rc = build_shared_edges(E, m->tri, m->neighbour, se.data(), err);
@@ -429,8 +447,8 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
 
     // device
     c->dev = prm->device;
-    HIPCHK(c, hipSetDevice(c->dev));
-    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CREATECHK(hipSetDevice(c->dev));
+    CREATECHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     size_t ubytes = sizeof(double) * (size_t)4 * c->Np * ld;
     // the kernels address every array with 32-bit byte offsets
     if (ubytes >= (1ull << 32) || sizeof(double) * (size_t)GEO_ROWS * ld >= (1ull << 32))
@@ -438,45 +456,45 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
         rkdg_destroy(c);
         return fail(nullptr, RKDG_ERR_ARG, "mesh too large for one context (state or geometry array above 4 GiB)");
     }
-    for (int b = 0; b < 3; ++b) HIPCHK(c, hipMalloc(&c->U[b], ubytes));
-    HIPCHK(c, hipMalloc(&c->geo, sizeof(double) * hgeo.size()));
-    HIPCHK(c, hipMalloc(&c->nbr, sizeof(int) * hnbr.size()));
-    HIPCHK(c, hipMalloc(&c->dtflag, hflag.size()));
-    HIPCHK(c, hipMalloc(&c->flags, sizeof(int) * ld));
-    HIPCHK(c, hipMalloc(&c->st, sizeof(DevState)));
+    for (int b = 0; b < 3; ++b) CREATECHK(hipMalloc(&c->U[b], ubytes));
+    CREATECHK(hipMalloc(&c->geo, sizeof(double) * hgeo.size()));
+    CREATECHK(hipMalloc(&c->nbr, sizeof(int) * hnbr.size()));
+    CREATECHK(hipMalloc(&c->dtflag, hflag.size()));
+    CREATECHK(hipMalloc(&c->flags, sizeof(int) * ld));
+    CREATECHK(hipMalloc(&c->st, sizeof(DevState)));
     c->bytes = 3 * ubytes + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size() + sizeof(int) * ld;
-    for (int b = 0; b < 3; ++b) HIPCHK(c, hipMemsetAsync(c->U[b], 0, ubytes, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->flags, 0, sizeof(int) * ld, c->stream));
+    for (int b = 0; b < 3; ++b) CREATECHK(hipMemsetAsync(c->U[b], 0, ubytes, c->stream));
+    CREATECHK(hipMemsetAsync(c->flags, 0, sizeof(int) * ld, c->stream));
     // host images are [rows][ld]; the device layout is tile-major
     const std::vector<double> tgeo = to_tile_major(hgeo, GEO_ROWS, ld);
     const std::vector<int> tnbr = to_tile_major(hnbr, NBR_ROWS, ld);
-    HIPCHK(c, hipMemcpyAsync(c->geo, tgeo.data(), sizeof(double) * tgeo.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->nbr, tnbr.data(), sizeof(int) * tnbr.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice, c->stream));
+    CREATECHK(hipMemcpyAsync(c->geo, tgeo.data(), sizeof(double) * tgeo.size(), hipMemcpyHostToDevice, c->stream));
+    CREATECHK(hipMemcpyAsync(c->nbr, tnbr.data(), sizeof(int) * tnbr.size(), hipMemcpyHostToDevice, c->stream));
+    CREATECHK(hipMemcpyAsync(c->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice, c->stream));
     if (c->staged) {
-        HIPCHK(c, hipMalloc(&c->nloc, sizeof(int) * hnloc.size()));
-        HIPCHK(c, hipMalloc(&c->hoff, sizeof(int) * hoff.size()));
-        HIPCHK(c, hipMalloc(&c->hids, sizeof(int) * std::max<size_t>(1, hids.size())));
+        CREATECHK(hipMalloc(&c->nloc, sizeof(int) * hnloc.size()));
+        CREATECHK(hipMalloc(&c->hoff, sizeof(int) * hoff.size()));
+        CREATECHK(hipMalloc(&c->hids, sizeof(int) * std::max<size_t>(1, hids.size())));
         c->bytes += sizeof(int) * (hnloc.size() + hoff.size() + hids.size());
         const std::vector<int> tnloc = to_tile_major(hnloc, NBR_ROWS, ld);
-        HIPCHK(c, hipMemcpyAsync(c->nloc, tnloc.data(), sizeof(int) * tnloc.size(), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->hoff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice, c->stream));
+        CREATECHK(hipMemcpyAsync(c->nloc, tnloc.data(), sizeof(int) * tnloc.size(), hipMemcpyHostToDevice, c->stream));
+        CREATECHK(hipMemcpyAsync(c->hoff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice, c->stream));
         if (!hids.empty())
-            HIPCHK(c, hipMemcpyAsync(c->hids, hids.data(), sizeof(int) * hids.size(), hipMemcpyHostToDevice, c->stream));
+            CREATECHK(hipMemcpyAsync(c->hids, hids.data(), sizeof(int) * hids.size(), hipMemcpyHostToDevice, c->stream));
     }
     if (prm->limiter == RKDG_LIMITER_TVB) {
-        HIPCHK(c, hipMalloc(&c->lim, sizeof(double) * hlim.size()));
-        HIPCHK(c, hipMalloc(&c->pairs, sizeof(int) * hpairs.size()));
+        CREATECHK(hipMalloc(&c->lim, sizeof(double) * hlim.size()));
+        CREATECHK(hipMalloc(&c->pairs, sizeof(int) * hpairs.size()));
         c->bytes += sizeof(double) * hlim.size() + sizeof(int) * hpairs.size();
         const std::vector<double> tlim = to_tile_major(hlim, LIM_ROWS, ld);
-        HIPCHK(c, hipMemcpyAsync(c->lim, tlim.data(), sizeof(double) * tlim.size(), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->pairs, hpairs.data(), sizeof(int) * hpairs.size(), hipMemcpyHostToDevice, c->stream));
+        CREATECHK(hipMemcpyAsync(c->lim, tlim.data(), sizeof(double) * tlim.size(), hipMemcpyHostToDevice, c->stream));
+        CREATECHK(hipMemcpyAsync(c->pairs, hpairs.data(), sizeof(int) * hpairs.size(), hipMemcpyHostToDevice, c->stream));
     }
     int rc = upload_tables(c->P, ref_basis(c->P).tab);
     if (rc) { std::string m2 = hip_error_string(rc); rkdg_destroy(c); return fail(nullptr, RKDG_ERR_HIP, "table upload: " + m2); }
     rc = launch_state_init(c->st, 0, 0.0, 1, c->stream);
-    if (rc) { rkdg_destroy(c); return fail(nullptr, RKDG_ERR_HIP, "state init"); }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (rc) { std::string m2 = hip_error_string(rc); rkdg_destroy(c); return fail(nullptr, RKDG_ERR_HIP, "state init: " + m2); }
+    CREATECHK(hipStreamSynchronize(c->stream));
     *out = c;
     return RKDG_OK;
 }
@@ -631,7 +649,10 @@ static int build_graph(rkdg_ctx* c) {
     int rc = launch_step(c, 0, 0.0);
     if (!rc) rc = launch_step(c, 1, 0.0);
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-    if (rc) return rc;
+    if (rc) {
+        if (e2 == hipSuccess) hipGraphDestroy(g);
+        return rc;
+    }
     if (e2 != hipSuccess) return fail(c, RKDG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e2));
     hipError_t e3 = hipGraphInstantiate(&c->graph2, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);

@@ -171,19 +171,19 @@ def test_batch_equals_single(P, staged, monkeypatch):
     for i in range(3):
         params[i] = make_params(degree=P, dt_policy="max", cfl=cfl0 + 0.02 * i)
     h = ctypes.c_void_p()
-    L.check(lib.rkdg_batch_create(3, meshes, params, ctypes.byref(h)), None, "batch_create")
+    L.bcheck(lib.rkdg_batch_create(3, meshes, params, ctypes.byref(h)), None, "batch_create")
     vp = np.array(vortex_params(), dtype=np.float64)
     for i in range(3):
-        L.check(lib.rkdg_batch_project_ic(h, i, 1, L.dptr(vp), len(vp)), None, "project")
-    L.check(lib.rkdg_batch_advance(h, 9, 1), None, "advance")
-    L.check(lib.rkdg_batch_sync(h), None, "sync")
+        L.bcheck(lib.rkdg_batch_project_ic(h, i, 1, L.dptr(vp), len(vp)), h, "project")
+    L.bcheck(lib.rkdg_batch_advance(h, 9, 1), h, "advance")
+    L.bcheck(lib.rkdg_batch_sync(h), h, "sync")
     for i, gr in enumerate(grids):
         s = Solver(gr, degree=P, dt_policy="max", cfl=cfl0 + 0.02 * i)
         s.project_ic(1, vortex_params())
         s.advance(9)
         Np = (P + 1) * (P + 2) // 2
         Ub = np.empty((4, Np, gr.n_elements))
-        L.check(lib.rkdg_batch_get_state(h, i, 1, L.dptr(Ub)), None, "get")
+        L.bcheck(lib.rkdg_batch_get_state(h, i, 1, L.dptr(Ub)), h, "get")
         # both are FP64 forms of the same scheme; they may fuse multiply-adds differently
         Us = s.get_state(basis=1)
         for v in range(4):
@@ -352,18 +352,18 @@ def test_precision_form_batch_equals_single(P):
     for i in range(2):
         params[i] = make_params(degree=P, dt_policy="max", cfl=cfl, precision="fp32")
     h = ctypes.c_void_p()
-    L.check(lib.rkdg_batch_create(2, meshes, params, ctypes.byref(h)), None, "batch_create")
+    L.bcheck(lib.rkdg_batch_create(2, meshes, params, ctypes.byref(h)), None, "batch_create")
     vp = np.array(vortex_params(), dtype=np.float64)
     for i in range(2):
-        L.check(lib.rkdg_batch_project_ic(h, i, 1, L.dptr(vp), len(vp)), None, "project")
-    L.check(lib.rkdg_batch_advance(h, 6, 1), None, "advance")
-    L.check(lib.rkdg_batch_sync(h), None, "sync")
+        L.bcheck(lib.rkdg_batch_project_ic(h, i, 1, L.dptr(vp), len(vp)), h, "project")
+    L.bcheck(lib.rkdg_batch_advance(h, 6, 1), h, "advance")
+    L.bcheck(lib.rkdg_batch_sync(h), h, "sync")
     for i, gr in enumerate(grids):
         s = Solver(gr, degree=P, dt_policy="max", cfl=cfl, precision="fp32")
         s.project_ic(1, vortex_params())
         s.advance(6)
         Ub = np.empty((4, (P + 1) * (P + 2) // 2, gr.n_elements))
-        L.check(lib.rkdg_batch_get_state(h, i, 1, L.dptr(Ub)), None, "get")
+        L.bcheck(lib.rkdg_batch_get_state(h, i, 1, L.dptr(Ub)), h, "get")
         Us = s.get_state(basis=1)
         if P <= 2:
             # both run the same staged MFMA tile kernel on the same tiles

@@ -276,3 +276,41 @@ def test_create_rejects_clockwise_elements():
     with pytest.raises(L.RkdgError) as ei:
         Solver(g, degree=1)
     assert ei.value.code == L.ERR_MESH and "clockwise" in str(ei.value)
+
+
+def _batch_create(grids, **kw):
+    from rkdg_heter_calculate_amd.solver import make_params, mesh_view
+    lib = L.load()
+    views = [mesh_view(g) for g in grids]
+    meshes = (L.Mesh * len(grids))(*views)
+    params = (L.Params * len(grids))(*[make_params(**kw) for _ in grids])
+    h = ctypes.c_void_p()
+    rc = lib.rkdg_batch_create(len(grids), meshes, params, ctypes.byref(h))
+    msg = lib.rkdg_batch_last_error(None)
+    msg = msg.decode() if msg else ""
+    if rc == 0:
+        lib.rkdg_batch_destroy(h)
+    return rc, msg
+
+
+@pytest.mark.parametrize("fault", ["shared_edge", "asymmetric", "tag"])
+def test_create_rejects_inconsistent_connectivity(fault):
+    """A shared-edge index outside 0..2, a one-sided neighbour relation or an unknown
+    boundary tag is refused by rkdg_create and rkdg_batch_create before the device is
+    touched: the kernels would read another element's traces or an undefined ghost."""
+    from rkdg_heter_calculate_amd import Solver
+    g = Grid.splitquad(4, 4, periodic=False)
+    e, k = next((e, k) for e in range(g.n_elements) for k in range(3) if g.neighbour[e, k] >= 0)
+    if fault == "shared_edge":
+        g.shared_edge[e, k] = 3
+    elif fault == "asymmetric":
+        n = g.neighbour[e, k]
+        g.neighbour[e, k] = next(x for x in range(g.n_elements) if x not in (e, n))
+    else:
+        eb, kb = next((e, k) for e in range(g.n_elements) for k in range(3) if g.neighbour[e, k] < 0)
+        g.boundary[eb, kb] = 7
+    with pytest.raises(L.RkdgError) as ei:
+        Solver(g, degree=1, dt_policy="max", cfl=0.3)
+    assert ei.value.code == L.ERR_MESH
+    rc, msg = _batch_create([Grid.splitquad(3, 3, periodic=True), g], degree=1, dt_policy="max", cfl=0.3)
+    assert rc == L.ERR_MESH, (rc, msg)
