@@ -83,7 +83,10 @@ static int upload_tiles(rkdg_batch* b) {
         for (long long t = b->off[i] / b->block; t < (b->off[i] + pad) / b->block; ++t)
             tm[t] = (i << 1) | (t * b->block == b->off[i] ? 1 : 0);
     }
+    // kernels and graph replays in flight read the old tile table
+    BCHK(b, hipStreamSynchronize(b->stream));
     if (b->tile_mesh) hipFree(b->tile_mesh);
+    b->tile_mesh = nullptr;
     BCHK(b, hipMalloc(&b->tile_mesh, sizeof(int) * tm.size()));
     BCHK(b, hipMemcpy(b->tile_mesh, tm.data(), sizeof(int) * tm.size(), hipMemcpyHostToDevice));
     if (b->graph2) { hipGraphExecDestroy(b->graph2); b->graph2 = nullptr; }
@@ -524,9 +527,14 @@ int rkdg_batch_autotune(rkdg_batch* b, int32_t* best_block) {
             // stage 1 reads buffers 1 and 0 and writes buffer 2: repeatable without
             // changing the state
             StageArgs a = bargs(b, 1, (int)(b->steps & 1));
-            launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, a, b->stream);
+            int lrc = launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, a, b->stream);
             BCHK(b, hipEventRecord(e0, b->stream));
-            for (int r = 0; r < 10; ++r) launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, a, b->stream);
+            for (int r = 0; r < 10 && !lrc; ++r) lrc = launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, a, b->stream);
+            if (lrc) {
+                hipEventDestroy(e0);
+                hipEventDestroy(e1);
+                return bfail(b, RKDG_ERR_HIP, std::string("autotune stage: ") + hip_error_string(lrc));
+            }
             BCHK(b, hipEventRecord(e1, b->stream));
             BCHK(b, hipEventSynchronize(e1));
             float ms;

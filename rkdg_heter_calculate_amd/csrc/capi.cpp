@@ -11,12 +11,13 @@
 //   st      DevState (wave-speed maxima, time, dt per step parity)
 #include <hip/hip_runtime.h>
 #include <algorithm>
-#include <atomic>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -787,11 +788,13 @@ int rkdg_time_stage(rkdg_ctx* c, int32_t s, int32_t reps, double* avg_ms) {
     HIPCHK(c, hipEventCreate(&b));
     // warm-up
     StageArgs args = make_args(c, s, slot, DTM_HOST, dt, 0.0);
-    launch_stage(c->P, c->prm.flux, c->prm.precision, c->block, args, c->stream);
+    int lrc = launch_stage(c->P, c->prm.flux, c->prm.precision, c->block, args, c->stream);
     HIPCHK(c, hipEventRecord(a, c->stream));
-    for (int r = 0; r < reps; ++r) {
-        int lrc = launch_stage(c->P, c->prm.flux, c->prm.precision, c->block, args, c->stream);
-        if (lrc) return fail(c, RKDG_ERR_HIP, hip_error_string(lrc));
+    for (int r = 0; r < reps && !lrc; ++r) lrc = launch_stage(c->P, c->prm.flux, c->prm.precision, c->block, args, c->stream);
+    if (lrc) {
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        return fail(c, RKDG_ERR_HIP, hip_error_string(lrc));
     }
     HIPCHK(c, hipEventRecord(b, c->stream));
     HIPCHK(c, hipEventSynchronize(b));
@@ -868,13 +871,11 @@ int rkdg_error_vs_exact(rkdg_ctx* c, int32_t kind, const double* p, int32_t np, 
     if (rc) return rc;
     const TriRule& R = projection_rule();
     const std::vector<double>& psi = projection_psi(P);
-    const int nt = 16;
-    std::vector<double> acc((size_t)nt * 12, 0.0);
-    std::vector<int> chunk_id(1, 0);
-    std::atomic<int> next{0};
+    // per-chunk sums (any number of host threads), merged in chunk order
+    std::vector<std::pair<int, std::array<double, 12>>> parts;
+    std::mutex parts_lock;
     parallel_for(E, [&](int a, int b) {
-        int id = next++;
-        double* s = &acc[(size_t)(id % nt) * 12];
+        double s[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int e = a; e < b; ++e) {
             const double* x = &c->ex[3 * e];
             const double* y = &c->ey[3 * e];
@@ -894,17 +895,22 @@ int rkdg_error_vs_exact(rkdg_ctx* c, int32_t kind, const double* p, int32_t np, 
                 }
             }
         }
+        std::array<double, 12> part;
+        std::copy(s, s + 12, part.begin());
+        std::lock_guard<std::mutex> g(parts_lock);
+        parts.emplace_back(a, part);
     });
-    double num[4] = {0, 0, 0, 0}, den[4] = {0, 0, 0, 0}, mx[4] = {0, 0, 0, 0};
-    for (int t = 0; t < nt; ++t)
+    std::sort(parts.begin(), parts.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (const auto& pt : parts)
         for (int v = 0; v < 4; ++v) {
-            num[v] += acc[(size_t)t * 12 + v];
-            den[v] += acc[(size_t)t * 12 + 4 + v];
-            mx[v] = std::max(mx[v], acc[(size_t)t * 12 + 8 + v]);
+            acc[v] += pt.second[v];
+            acc[4 + v] += pt.second[4 + v];
+            acc[8 + v] = std::max(acc[8 + v], pt.second[8 + v]);
         }
     for (int v = 0; v < 4; ++v) {
-        out[v] = den[v] > 0 ? std::sqrt(num[v] / den[v]) : std::sqrt(num[v]);
-        out[4 + v] = mx[v];
+        out[v] = acc[4 + v] > 0 ? std::sqrt(acc[v] / acc[4 + v]) : std::sqrt(acc[v]);
+        out[4 + v] = acc[8 + v];
     }
     return RKDG_OK;
 }

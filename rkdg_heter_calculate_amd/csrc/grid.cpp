@@ -2,7 +2,9 @@
 // locality order.  Replaces the reference's readers and its O(N^2) neighbour/ghost
 // set-up (src/unstructuredgrid.cpp:14-80,88-107,193-305,502-643,729-760,934-976).
 #include <algorithm>
+#include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
