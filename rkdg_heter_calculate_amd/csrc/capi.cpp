@@ -113,6 +113,7 @@ struct rkdg_ctx {
     int* pairs = nullptr;
     int* flags = nullptr;
     DevState* st = nullptr;
+    double* res_part = nullptr;  // residual maxima per tile [ntiles][2], written by stage 2
     int cur = 0;            // buffer holding the latest state
     int next_stage = 0;     // rkdg_stage protocol
     long long steps = 0;    // steps launched (host count; parity of the step slot)
@@ -168,6 +169,7 @@ static StageArgs make_args(rkdg_ctx* c, int stage, int slot, int dt_mode, double
     a.pipe = c->pipe;
     a.dtflag = c->dtflag;
     a.st = c->st;
+    a.res_part = c->res_part;
     a.ld = c->ld;
     a.E = c->E;
     a.stage = stage;
@@ -463,9 +465,12 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     CREATECHK(hipMalloc(&c->dtflag, hflag.size()));
     CREATECHK(hipMalloc(&c->flags, sizeof(int) * ld));
     CREATECHK(hipMalloc(&c->st, sizeof(DevState)));
-    c->bytes = 3 * ubytes + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size() + sizeof(int) * ld;
+    CREATECHK(hipMalloc(&c->res_part, sizeof(double) * 2 * c->ntiles));
+    c->bytes = 3 * ubytes + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size() + sizeof(int) * ld +
+               sizeof(double) * 2 * c->ntiles;
     for (int b = 0; b < 3; ++b) CREATECHK(hipMemsetAsync(c->U[b], 0, ubytes, c->stream));
     CREATECHK(hipMemsetAsync(c->flags, 0, sizeof(int) * ld, c->stream));
+    CREATECHK(hipMemsetAsync(c->res_part, 0, sizeof(double) * 2 * c->ntiles, c->stream));
     // host images are [rows][ld]; the device layout is tile-major
     const std::vector<double> tgeo = to_tile_major(hgeo, GEO_ROWS, ld);
     const std::vector<int> tnbr = to_tile_major(hnbr, NBR_ROWS, ld);
@@ -508,7 +513,7 @@ int rkdg_destroy(rkdg_ctx* c) {
     for (int b = 0; b < 3; ++b) hipFree(c->U[b]);
     hipFree(c->geo); hipFree(c->nbr); hipFree(c->dtflag); hipFree(c->lim); hipFree(c->pairs);
     hipFree(c->nloc); hipFree(c->hoff); hipFree(c->hids);
-    hipFree(c->flags); hipFree(c->st);
+    hipFree(c->flags); hipFree(c->st); hipFree(c->res_part);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return RKDG_OK;
@@ -739,11 +744,15 @@ int rkdg_advance(rkdg_ctx* c, int32_t nsteps, double t_end, int32_t* steps_done,
 int rkdg_residual(rkdg_ctx* c, double res[2]) {
     if (!c || !res) return fail(c, RKDG_ERR_ARG, "residual");
     HIPCHK(c, hipSetDevice(c->dev));
-    DevState h;
-    HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    // the last stage 2 left one pair of maxima per tile
+    std::vector<double> h((size_t)2 * c->ntiles);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->res_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::memcpy(&res[0], &h.res[0], sizeof(double));
-    std::memcpy(&res[1], &h.res[1], sizeof(double));
+    res[0] = res[1] = 0.0;
+    for (int t = 0; t < c->ntiles; ++t) {
+        res[0] = std::max(res[0], h[(size_t)2 * t]);
+        res[1] = std::max(res[1], h[(size_t)2 * t + 1]);
+    }
     return RKDG_OK;
 }
 

@@ -76,7 +76,6 @@ struct DevState {
     unsigned long long ws[2];      // max (|u|+c) P/A as double bits (non-negative)
     double t[2];
     double dt[2];
-    unsigned long long res[2];     // calculateResidual maxima (bits)
     int err_pos;                   // smallest device position with non-physical mean (INT_MAX: none)
     int pad;
     long long nsteps;              // steps advanced (end-time no-ops excluded)
@@ -158,6 +157,7 @@ struct StageArgs {
     const int* nbr;        // [3][ld] packed: (pos << 2) | shared edge, or -tag for a boundary
     const unsigned char* dtflag;
     DevState* st;
+    double* res_part;      // stage 2: residual maxima per tile [ntiles][2] (NULL: not kept)
     const int* tile_mesh;  // batching: mesh id per workgroup tile (NULL: single mesh)
     DevState* st_batch;    // batching: per-mesh states
     const double* cfl_batch;

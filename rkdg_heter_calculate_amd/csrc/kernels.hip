@@ -74,7 +74,6 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Block-wide max of three non-negative values, then one atomic per value.
 // Raises *p (the bits of a non-negative double, which order like the values) to x.  The
 // atomic is skipped when the current value already covers x: one workgroup per tile would
 // otherwise queue tens of thousands of atomics on one address per stage.  The final value is
@@ -85,22 +84,23 @@ __device__ __forceinline__ void atomic_raise(unsigned long long* p, double x) {
     if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bits) atomicMax(p, bits);
 }
 
-__device__ __forceinline__ void block_max3_atomic(double a, double b, double c, unsigned long long* pa,
-                                                  unsigned long long* pb, unsigned long long* pc, bool do_bc) {
+// Workgroup maxima of stage 2: the CFL wave speed ws is raised into *pws (one atomic at
+// most); the two residual maxima r0, r1 are stored as this tile's pair rpart[0..1] with
+// plain stores (NULL: not kept) and max-reduced on the host when asked for.  Device-wide
+// atomics cost about 6 us per residual per stage at 0.5M triangles; the stores cost nothing.
+__device__ __forceinline__ void block_stage_max(double ws, double r0, double r1, unsigned long long* pws,
+                                                double* rpart) {
     __shared__ double red[3][16];
-    a = wave_max(a);
-    if (do_bc) { b = wave_max(b); c = wave_max(c); }
+    ws = wave_max(ws);
+    if (rpart) { r0 = wave_max(r0); r1 = wave_max(r1); }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    if (lane == 0) { red[0][wid] = a; red[1][wid] = b; red[2][wid] = c; }
+    if (lane == 0) { red[0][wid] = ws; red[1][wid] = r0; red[2][wid] = r1; }
     __syncthreads();
     if (threadIdx.x == 0) {
         double ma = 0, mb = 0, mc = 0;
         for (int w = 0; w < nw; ++w) { ma = fmax(ma, red[0][w]); mb = fmax(mb, red[1][w]); mc = fmax(mc, red[2][w]); }
-        atomic_raise(pa, ma);
-        if (do_bc) {
-            atomic_raise(pb, mb);
-            atomic_raise(pc, mc);
-        }
+        atomic_raise(pws, ma);
+        if (rpart) { rpart[0] = mb; rpart[1] = mc; }
     }
 }
 
@@ -258,6 +258,11 @@ __device__ __forceinline__ double stage_dt(const StageArgs& a, const TileInfo& t
         else if (rem < dt) dt = rem;
     }
     return dt;
+}
+
+// This tile's pair of residual maxima (stage 2), or NULL when they are not kept.
+__device__ __forceinline__ double* tile_residual(const StageArgs& a, const TileInfo& ti) {
+    return a.res_part ? a.res_part + 2 * ti.tile : nullptr;
 }
 
 // Neighbour prefetch: the neighbour's modes are loaded one edge ahead (during the volume
@@ -623,8 +628,6 @@ __device__ __forceinline__ void stage_bookkeeping(const StageArgs& a, const Tile
     if (ti.first && threadIdx.x == 0) {
         if (a.stage == 0 && !done) {
             ti.st->ws[1 - a.slot] = 0ull;
-            ti.st->res[0] = 0ull;
-            ti.st->res[1] = 0ull;
             ti.st->dt[a.slot] = dt;
         }
         if (a.stage == 2) {
@@ -748,7 +751,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
         element_core<P, FLUX, STAGED ? NB_LDS : NB_GLOBAL>(a, e, Uo, o.eg, o.w0, o.w1, o.w2, dt, o.dtf, sVe, sU,
                                                           ti.st, ws_l, r0_l, r1_l);
     if (a.stage == 2)
-        block_max3_atomic(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], &ti.st->res[0], &ti.st->res[1], true);
+        block_stage_max(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], tile_residual(a, ti));
 }
 
 // Persistent staged form with the next tile's image in flight during the current tile's
@@ -811,7 +814,7 @@ __global__ void __launch_bounds__(256) k_stage_pipe(StageArgs a) {
         double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
         if (live) element_update<P, NB_LDS_LEAN>(a, e, Uo, o.eg, W, dt, o.dtf, ti.st, ws_l, r0_l, r1_l);
         if (a.stage == 2)
-            block_max3_atomic(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], &ti.st->res[0], &ti.st->res[1], true);
+            block_stage_max(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], tile_residual(a, ti));
         ti = tn;
         o = on;
         have = have_n;
@@ -876,7 +879,7 @@ __global__ void __launch_bounds__(256) k_wavespeed(StageArgs a) {
             const double s = (sqrt(u * u + v * v) + c) * per / eg.area;
             if (a.dtflag[e] && s > 0.0) s_l = s;
         }
-        block_max3_atomic(s_l, 0.0, 0.0, &ti.st->ws[a.slot], nullptr, nullptr, false);
+        block_stage_max(s_l, 0.0, 0.0, &ti.st->ws[a.slot], nullptr);
         __syncthreads();
     }
 }
@@ -887,7 +890,6 @@ __global__ void k_state_init(DevState* st, int n, int slot, double t) {
         st[i].ws[0] = 0ull; st[i].ws[1] = 0ull;
         st[i].t[slot] = t; st[i].t[1 - slot] = t;
         st[i].dt[0] = 0.0; st[i].dt[1] = 0.0;
-        st[i].res[0] = 0ull; st[i].res[1] = 0ull;
         st[i].err_pos = INT_MAX;
         st[i].nsteps = 0;
     }

@@ -90,6 +90,8 @@ def test_boundaries_match_oracle(P, prec):
         o.advance(8)
         err = rel_l2(s.get_state(), o.get_state(), o.mass())
         assert np.all(err < (TOL if prec == "fp64" else MX_TOL)), (flux, err)
+        # the reference's residual (max change of the rho and rhoE means), reduced over tiles
+        assert np.allclose(s.residual(), o.residual_change(), rtol=1e-9 if prec == "fp64" else 1e-3, atol=1e-15)
 
 
 def test_limiter_matches_oracle():
@@ -308,6 +310,7 @@ def test_precision_form_staged_tiles_with_halos(P, flux):
     print("relative L2 vs FP64 oracle", err)
     assert np.all(err < MX_TOL), err
     assert np.all(err > 0)
+    assert np.allclose(s.residual(), o.residual_change(), rtol=1e-3, atol=1e-15)
     # the four-lanes-per-element kernel (RKDG_PX=0) rounds differently: a different result
     # shows that the staged lane-per-element kernel ran
     import os
