@@ -64,47 +64,48 @@ int rkdg_write_plt(const char* path, const rkdg_grid* g, int32_t Np, const doubl
     if (!path || !g || !U || Np < 1) return RKDG_ERR_ARG;
     std::ofstream f(path);
     if (!f) return RKDG_ERR_IO;
+    // '\n' rather than std::endl: the same bytes without a flush per line
     const int nv = g->n_vertices, ne = g->n_elements;
     auto mean = [&](int v, int e) { return U[((size_t)v * Np) * ne + e]; };
-    f << "TITLE=RKDG" << std::endl;
-    f << "VARIABLES=X , Y , rho , u , v , p, Ma , FLAG" << std::endl;
-    f << "ZONE T= T1 N= " << nv << " , E= " << ne << " , ZONETYPE=FETRIANGLE" << std::endl;
-    f << "DATAPACKING=BLOCK" << std::endl;
-    f << "VARLOCATION=([1-2]=NODAL,[3-8]=CELLCENTERED)" << std::endl;
-    f << "DT=(SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE)" << std::endl;
+    f << "TITLE=RKDG" << '\n';
+    f << "VARIABLES=X , Y , rho , u , v , p, Ma , FLAG" << '\n';
+    f << "ZONE T= T1 N= " << nv << " , E= " << ne << " , ZONETYPE=FETRIANGLE" << '\n';
+    f << "DATAPACKING=BLOCK" << '\n';
+    f << "VARLOCATION=([1-2]=NODAL,[3-8]=CELLCENTERED)" << '\n';
+    f << "DT=(SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE SINGLE)" << '\n';
     auto block = [&](int n, auto value) {
         for (int i = 0; i < n; ++i) {
             value(i);
             f << "  ";
-            if (i % 6 == 0) f << std::endl;
+            if (i % 6 == 0) f << '\n';
         }
     };
     block(nv, [&](int i) { f << g->vx[i]; });
     block(nv, [&](int i) { f << g->vy[i]; });
-    f << std::endl;
+    f << '\n';
     block(ne, [&](int i) { f << mean(0, i); });
-    f << std::endl;
+    f << '\n';
     block(ne, [&](int i) { f << mean(1, i) / mean(0, i); });
-    f << std::endl;
+    f << '\n';
     block(ne, [&](int i) { f << mean(2, i) / mean(0, i); });
-    f << std::endl;
+    f << '\n';
     auto prim = [&](int i, double& rho, double& u, double& v, double& p) {
         rho = mean(0, i); u = mean(1, i) / rho; v = mean(2, i) / rho;
         p = (gamma - 1) * (mean(3, i) - 0.5 * rho * (u * u + v * v));
     };
     block(ne, [&](int i) { double r, u, v, p; prim(i, r, u, v, p); f << p; });
-    f << std::endl;
+    f << '\n';
     block(ne, [&](int i) {
         double r, u, v, p;
         prim(i, r, u, v, p);
         double a = std::sqrt(gamma * p / r);
         f << std::sqrt(u * u + v * v) / a;
     });
-    f << std::endl;
+    f << '\n';
     block(ne, [&](int i) { f << (flags ? flags[i] : 1); });
-    f << std::endl;
+    f << '\n';
     for (int i = 0; i < ne; ++i)
-        f << g->tri[3 * i] + 1 << "    " << g->tri[3 * i + 1] + 1 << "    " << g->tri[3 * i + 2] + 1 << std::endl;
+        f << g->tri[3 * i] + 1 << "    " << g->tri[3 * i + 1] + 1 << "    " << g->tri[3 * i + 2] + 1 << '\n';
     return f ? RKDG_OK : RKDG_ERR_IO;
 }
 
@@ -142,7 +143,7 @@ int rkdg_dump_state(const char* path, int32_t Np, int32_t n_elements, double t, 
 }
 
 int rkdg_load_state(const char* path, int32_t Np, int32_t n_elements, double* t, double* U) {
-    if (!path || !U) return RKDG_ERR_ARG;
+    if (!path || !U || Np < 1 || n_elements < 1) return RKDG_ERR_ARG;
     FILE* f = std::fopen(path, "rb");
     if (!f) return RKDG_ERR_IO;
     char mg[8];
