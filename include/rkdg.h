@@ -153,8 +153,17 @@ int rkdg_sync(rkdg_ctx* ctx);
  * context's stream; returns the average per launch in ms.  For measurement only. */
 int rkdg_time_stage(rkdg_ctx* ctx, int32_t s, int32_t reps, double* avg_ms);
 
-/* Times `nsteps` steps of rkdg_advance with HIP events on the context's stream. */
-int rkdg_time_advance(rkdg_ctx* ctx, int32_t nsteps, double* total_ms);
+/* Times `nsteps` steps launched stage by stage, with HIP events on the context's stream
+ * around the whole loop and around every stage (and limiter) launch.  out[0]: total ms;
+ * out[1]: mean ms of a stage launch as bracketed by its own events (includes the dispatch
+ * of each launch); out[2]: stage launches; out[3]: mean ms of a limiter launch (0 without
+ * limiter).  For measurement only (no reference counterpart). */
+int rkdg_time_advance(rkdg_ctx* ctx, int32_t nsteps, double* out);
+
+/* Times `nsteps` steps of rkdg_advance (device dt, hipGraph-replayed step pairs) with one
+ * HIP event before and one after, on the context's stream; *total_ms is the elapsed time.
+ * For measurement only (no reference counterpart). */
+int rkdg_time_steps(rkdg_ctx* ctx, int32_t nsteps, double* total_ms);
 
 /* Error of the current state against the exact solution `kind` (RKDG_IC_VORTEX with its
  * t parameter set, or RKDG_IC_FREESTREAM), on the 36-point rule per element:

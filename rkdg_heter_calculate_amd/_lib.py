@@ -72,6 +72,7 @@ SIGNATURES = [
     ("rkdg_sync", c_int32, [ctypes.c_void_p]),
     ("rkdg_time_stage", c_int32, [ctypes.c_void_p, c_int32, c_int32, P_double]),
     ("rkdg_time_advance", c_int32, [ctypes.c_void_p, c_int32, P_double]),
+    ("rkdg_time_steps", c_int32, [ctypes.c_void_p, c_int32, P_double]),
     ("rkdg_error_vs_exact", c_int32, [ctypes.c_void_p, c_int32, P_double, c_int32, P_double]),
     ("rkdg_measure_copy", c_int32, [c_int32, c_int64, c_int32, P_double]),
     ("rkdg_info", c_int32, [ctypes.c_void_p, P_int64]),

@@ -872,6 +872,32 @@ int rkdg_time_advance(rkdg_ctx* c, int32_t nsteps, double* out) {
     return check_physical(c);
 }
 
+int rkdg_time_steps(rkdg_ctx* c, int32_t nsteps, double* total_ms) {
+    if (!c || nsteps < 1 || !total_ms) return fail(c, RKDG_ERR_ARG, "time_steps");
+    if (!c->have_state) return fail(c, RKDG_ERR_STATE, "no state");
+    if (c->next_stage != 0) return fail(c, RKDG_ERR_STATE, "a step started with rkdg_stage is unfinished");
+    HIPCHK(c, hipSetDevice(c->dev));
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    hipError_t e = hipEventCreate(&b);
+    if (e != hipSuccess) { hipEventDestroy(a); return fail(c, RKDG_ERR_HIP, hipGetErrorString(e)); }
+    int rc = RKDG_OK;
+    e = hipEventRecord(a, c->stream);
+    if (e == hipSuccess) {
+        rc = advance_impl(c, nsteps, 0.0, nullptr, nullptr);
+        if (!rc) e = hipEventRecord(b, c->stream);
+    }
+    if (!rc && e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0;
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(c, RKDG_ERR_HIP, hipGetErrorString(e));
+    *total_ms = ms;
+    return check_physical(c);
+}
+
 int rkdg_error_vs_exact(rkdg_ctx* c, int32_t kind, const double* p, int32_t np, double out[8]) {
     if (!c || !p || !out || (kind != RKDG_IC_VORTEX && kind != RKDG_IC_FREESTREAM)) return fail(c, RKDG_ERR_ARG, "error_vs_exact");
     const int E = c->E, Np = c->Np, P = c->P;

@@ -247,6 +247,12 @@ class Solver:
         self._chk(self.lib.rkdg_time_advance(self.h, nsteps, L.dptr(out)), "time_advance")
         return {"total_ms": out[0], "stage_ms": out[1], "launches": int(out[2]), "limiter_ms": out[3]}
 
+    def time_steps(self, nsteps):
+        """Event-timed ms of `nsteps` steps of advance() (hipGraph-replayed, device dt)."""
+        ms = ctypes.c_double()
+        self._chk(self.lib.rkdg_time_steps(self.h, nsteps, ctypes.byref(ms)), "time_steps")
+        return ms.value
+
     def error_vs_exact(self, kind, params):
         p = np.ascontiguousarray(params, dtype=np.float64)
         out = np.zeros(8)

@@ -154,6 +154,25 @@ def test_advance_counts_steps_per_call():
     s.close()
 
 
+def test_timed_steps_equal_advance():
+    """The measurement entry point rkdg_time_steps takes exactly the steps of rkdg_advance
+    (graph-replayed pairs after an odd step count, device dt)."""
+    print("case: time_steps vs advance", flush=True)
+    g = vortex_grid(16, jitter=0.2, diagonal=1, seed=6)
+    a = Solver(g, degree=2, dt_policy="max", cfl=0.3)
+    b = Solver(g, degree=2, dt_policy="max", cfl=0.3)
+    for s in (a, b):
+        s.project_ic(1, vortex_params())
+        s.advance(1)
+    a.advance(7)
+    ms = b.time_steps(7)
+    assert ms > 0
+    assert np.array_equal(a.get_state(), b.get_state())
+    assert a.time() == b.time()
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("P,staged", [(2, False), (2, True), (1, True), (3, False)])
 def test_batch_equals_single(P, staged, monkeypatch):
     print(f"case: batch vs single P{P} staged={staged}", flush=True)
