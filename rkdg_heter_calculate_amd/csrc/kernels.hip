@@ -327,7 +327,7 @@ struct ElemWork {
 template <int P, int FLUX, int NBSRC>
 __device__ __forceinline__ void element_residual(const StageArgs& a, int e, const double (&Uo)[4][Deg<P>::Np],
                                                  const ElemGeo& eg, int w0, int w1, int w2, const double* sVe,
-                                                 const double* sU, ElemWork<P>& W) {
+                                                 const double* sU, ElemWork<P>& W, const double* sGe = nullptr) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqv = D::Nqv, Nqe = D::Nqe;
     constexpr bool PF = NBSRC == NB_GLOBAL && Prefetch<P>::on;
@@ -369,16 +369,17 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
 
     // ---- surface integral, element-owned -----------------------------------------
     // Both loops stay rolled: one quadrature point's traces and flux are live at a time.
-    // Edge k's direction s_k (edge_normal) for k = 1, 2 is re-read one edge ahead (L2 hits)
-    // rather than keeping the inverse Jacobian live across the loop.
+    // Edge k's direction s_k (edge_normal) for k = 1, 2 is re-read one edge ahead (L2 hits,
+    // or the LDS copy sGe: rows of 256 doubles) rather than keeping the inverse Jacobian
+    // live across the loop.
     double sx_n = eg.xix + eg.etx, sy_n = eg.xiy + eg.ety, per = 0.0;
 #pragma unroll 1
     for (int k = 0; k < 3; ++k) {
         const int w = k == 0 ? w0 : (k == 1 ? w1 : w2);
         const double sx = sx_n, sy = sy_n;
         if (k < 2) {
-            sx_n = -tmx(a.geo, 2 * k, GEO_ROWS, e);
-            sy_n = -tmx(a.geo, 2 * k + 1, GEO_ROWS, e);
+            sx_n = -(sGe ? sGe[(2 * k) * 256] : tmx(a.geo, 2 * k, GEO_ROWS, e));
+            sy_n = -(sGe ? sGe[(2 * k + 1) * 256] : tmx(a.geo, 2 * k + 1, GEO_ROWS, e));
         }
         double nx, ny, lk;
         edge_frame(sx, sy, eg.area, nx, ny, lk);
@@ -616,9 +617,9 @@ template <int P, int FLUX, int NBSRC>
 __device__ __forceinline__ void element_core(const StageArgs& a, int e, const double (&Uo)[4][Deg<P>::Np],
                                              const ElemGeo& eg, int w0, int w1, int w2, double dt, bool dtf,
                                              const double* sVe, const double* sU, DevState* st, double& ws_l,
-                                             double& r0_l, double& r1_l) {
+                                             double& r0_l, double& r1_l, const double* sGe = nullptr) {
     ElemWork<P> W;
-    element_residual<P, FLUX, NBSRC>(a, e, Uo, eg, w0, w1, w2, sVe, sU, W);
+    element_residual<P, FLUX, NBSRC>(a, e, Uo, eg, w0, w1, w2, sVe, sU, W, sGe);
     element_update<P, NBSRC>(a, e, Uo, eg, W, dt, dtf, st, ws_l, r0_l, r1_l);
 }
 
@@ -818,6 +819,144 @@ __global__ void __launch_bounds__(256) k_stage_pipe(StageArgs a) {
         ti = tn;
         o = on;
         have = have_n;
+    }
+}
+
+// Register-pipelined persistent staged form (P1): the next tile's own modes, halo modes and
+// operands are loaded into registers while the current tile computes its residual and
+// update, and written into the single LDS image at the top of the next iteration.  Loads
+// complete in order, so a wait for any load also waits for every load issued before it:
+// everything the current tile reads from memory (its dt, then the update's step-start
+// modes, which come last) is issued before the prefetch or long after it, and the halo's
+// indices run ahead of its data (the per-tile halo offsets three tiles ahead, the lane's
+// halo element id two tiles ahead), so no wait inside an iteration stops at the prefetch.
+// Halo lane layout: lane t holds halo element t % 128 (at most S - B = 96), rows t / 128 + 2j.
+template <int P>
+struct RpImage {
+    static constexpr int NR = 4 * Deg<P>::Np, NH = NR / 2;
+    double u[NR];    // own modes of the lane's element
+    double hx[NH];   // rows of the lane's halo element
+    double g[5];     // inverse Jacobian rows and area
+    int w[3];        // neighbour codes (LDS slots)
+    int dtf;         // dt-subset byte, raw (compared when used)
+    int nh;          // halo elements of the tile
+};
+
+template <int P>
+__device__ __forceinline__ void rp_data(const StageArgs& a, int tile, int nh, int hid, RpImage<P>& im) {
+    using L = Staged<P>;
+    constexpr int NR = RpImage<P>::NR, NH = RpImage<P>::NH;
+    const int tid = threadIdx.x, hh = tid & 127, hr = tid >> 7;
+    const int e = tile * L::B + tid;  // < ld: the last tile's padded columns lie inside the arrays
+    const __amdgpu_buffer_rsrc_t rU = tm_rsrc(a.U, a.ld, NR);
+    const unsigned vo = tm_voff(e, NR);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) im.u[r] = tm_load(rU, vo, r);
+    im.nh = nh;
+    if (hh < nh) {
+        const unsigned vh = tm_voff(hid, NR);
+#pragma unroll
+        for (int j = 0; j < NH; ++j) im.hx[j] = tm_load(rU, vh, hr + 2 * j);
+    }
+    // operands (also read for the padded lanes: their columns hold the padding's values)
+    const __amdgpu_buffer_rsrc_t rG = tm_rsrc(a.geo, a.ld, GEO_ROWS);
+    const unsigned vg = tm_voff(e, GEO_ROWS);
+#pragma unroll
+    for (int r = 0; r < GEO_ROWS; ++r) im.g[r] = tm_load(rG, vg, r);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) im.w[k] = tmx(a.nloc, k, NBR_ROWS, e);
+    im.dtf = a.dtflag[e];
+}
+
+// Lane's halo element id of a tile (0 when the lane holds none) from its halo offsets.
+__device__ __forceinline__ int rp_halo_id(const StageArgs& a, int2 ho) {
+    const int hh = threadIdx.x & 127;
+    return hh < ho.y - ho.x ? a.hids[ho.x + hh] : 0;
+}
+
+__device__ __forceinline__ int2 rp_halo_off(const StageArgs& a, int tile) {
+    return make_int2(a.hoff[tile], a.hoff[tile + 1]);
+}
+
+template <int P, int FLUX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_stage_rp(StageArgs a) {
+    using L = Staged<P>;
+    constexpr int Np = Deg<P>::Np, Nqe = Deg<P>::Nqe, NR = RpImage<P>::NR, NH = RpImage<P>::NH;
+    const DevTables& T = c_tab[P - 1];
+    __shared__ double sVe[3 * Nqe * Np];
+    extern __shared__ __align__(16) double sU[];
+    for (int t = threadIdx.x; t < 3 * Nqe * Np; t += blockDim.x) {
+        const int k = t / (Nqe * Np), r = t - k * (Nqe * Np), j = r / Np, i = r - j * Np;
+        sVe[t] = T.Ve[k][j][i];
+    }
+    const int nit = iter_count(a);
+    const int G = gridDim.x, tid = threadIdx.x, hh = tid & 127, hr = tid >> 7;
+    // tiles of this workgroup's iterations it, it + G, it + 2G, it + 3G (uniform)
+    int it = blockIdx.x;
+    TileInfo t0, t1, t2;
+    bool h0 = it < nit && tile_of_iter(a, it, nit, t0);
+    bool h1 = h0 && it + G < nit && tile_of_iter(a, it + G, nit, t1);
+    bool h2 = h1 && it + 2 * G < nit && tile_of_iter(a, it + 2 * G, nit, t2);
+    RpImage<P> im;
+    int2 ho1 = make_int2(0, 0), ho2 = make_int2(0, 0);
+    int hid1 = 0;
+    if (h0) {
+        const int2 ho0 = rp_halo_off(a, t0.tile);
+        rp_data<P>(a, t0.tile, ho0.y - ho0.x, rp_halo_id(a, ho0), im);
+    }
+    if (h1) {
+        ho1 = rp_halo_off(a, t1.tile);
+        hid1 = rp_halo_id(a, ho1);
+    }
+    if (h2) ho2 = rp_halo_off(a, t2.tile);
+    // one mesh: the step's dt and end-time state are the same for every tile of the launch
+    bool done = false;
+    const double dt = h0 ? stage_dt(a, t0, done) : 0.0;
+    while (h0) {  // uniform across the workgroup
+        // this tile's image from the registers (the previous tile is done with the LDS)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sU[r * L::S + tid] = im.u[r];
+        if (hh < im.nh) {
+#pragma unroll
+            for (int j = 0; j < NH; ++j) sU[(hr + 2 * j) * L::S + L::B + hh] = im.hx[j];
+        }
+        // the inverse Jacobian rows, re-read by the edge loop (a load from memory there would
+        // wait for the whole prefetch)
+        double* sG = sU + NR * L::S;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sG[r * L::B + tid] = im.g[r];
+        double Uo[4][Np];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 0; i < Np; ++i) Uo[v][i] = im.u[v * Np + i];
+        ElemOps o;
+        o.eg = ElemGeo{im.g[0], im.g[1], im.g[2], im.g[3], im.g[4]};
+        o.w0 = im.w[0]; o.w1 = im.w[1]; o.w2 = im.w[2];
+        o.dtf = a.stage == 2 && im.dtf != 0;
+        __syncthreads();  // the image (and sVe on entry) is complete
+        // prefetch: the next tile's data, the halo ids of the one after, the offsets of the third
+        TileInfo t3;
+        const bool h3 = h2 && it + 3 * G < nit && tile_of_iter(a, it + 3 * G, nit, t3);
+        if (h1) rp_data<P>(a, t1.tile, ho1.y - ho1.x, hid1, im);
+        int hid2 = 0;
+        if (h2) hid2 = rp_halo_id(a, ho2);
+        int2 ho3 = make_int2(0, 0);
+        if (h3) ho3 = rp_halo_off(a, t3.tile);
+        const int e = t0.tile * L::B + tid;
+        double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
+        if (e < a.E && !done)
+            element_core<P, FLUX, NB_LDS>(a, e, Uo, o.eg, o.w0, o.w1, o.w2, dt, o.dtf, sVe, sU, t0.st, ws_l, r0_l, r1_l,
+                                          sU + NR * L::S + tid);
+        stage_bookkeeping(a, t0, dt, done);
+        if (a.stage == 2 && !done)  // (done is uniform: one end time per mesh)
+            block_stage_max(ws_l, r0_l, r1_l, &t0.st->ws[1 - a.slot], tile_residual(a, t0));
+        __syncthreads();  // every wave is done with the image before the next one is written
+        it += G;
+        t0 = t1; t1 = t2; t2 = t3;
+        h0 = h1; h1 = h2; h2 = h3;
+        ho1 = ho2; ho2 = ho3;
+        hid1 = hid2;
     }
 }
 
@@ -1068,7 +1207,9 @@ static void launch_stage_pf(int block, const StageArgs& a, hipStream_t s) {
             hipFuncSetAttribute((const void*)k_stage_pipe<P, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             opted = true;
         }
-        if (a.pipe) hipLaunchKernelGGL((k_stage_pipe<P, FLUX>), grid, blk, lds, s, a);
+        if (P == 1 && a.pipe == 2 && !a.tile_mesh)  // + the LDS copy of 4 geometry rows
+            hipLaunchKernelGGL((k_stage_rp<1, FLUX>), grid, blk, lds + 4 * staged_block(1) * sizeof(double), s, a);
+        else if (a.pipe) hipLaunchKernelGGL((k_stage_pipe<P, FLUX>), grid, blk, lds, s, a);
         else hipLaunchKernelGGL((k_stage<P, FLUX, true>), grid, blk, lds, s, a);
         return;
     }

@@ -154,6 +154,33 @@ def test_advance_counts_steps_per_call():
     s.close()
 
 
+def test_p1_pipelined_tiles_match_oracle(monkeypatch):
+    """P1 LF FP64 runs the register-pipelined persistent tile (next tile's image loaded into
+    registers during the current tile). 528 full tiles and a partial one on 256 workgroups
+    (RKDG_PERSIST=1 on a 256-CU part) give each workgroup two or three tiles, so the
+    prefetch chain (data one tile ahead, halo ids two, halo offsets three) runs; the result
+    must equal the one-tile-per-workgroup staged kernel bit for bit and the oracle to 1e-12."""
+    print("case: P1 LF pipelined persistent tiles", flush=True)
+    g = Grid.splitquad(260, 260, 0.0, 0.0, 10.0, 10.0, periodic=True, diagonal=1, jitter=0.2, seed=21)
+    assert g.n_elements == 135200
+    monkeypatch.setenv("RKDG_PERSIST", "1")
+    o, s = make_pair(g, P=1, flux="lf", dt_policy="max", cfl=0.3)
+    monkeypatch.setenv("RKDG_PIPE", "0")
+    monkeypatch.delenv("RKDG_PERSIST")
+    r = Solver(g, degree=1, flux="lf", dt_policy="max", cfl=0.3)
+    o.project(1, vortex_params())
+    U0 = o.get_state()
+    s.set_state(U0)
+    r.set_state(U0)
+    s.advance(4)
+    r.advance(4)
+    o.advance(4)
+    assert np.array_equal(s.get_state(), r.get_state())
+    err = rel_l2(s.get_state(), o.get_state(), o.mass())
+    assert np.all(err < TOL), err
+    assert np.allclose(s.residual(), o.residual_change(), rtol=1e-9, atol=1e-15)
+
+
 def test_timed_steps_equal_advance():
     """The measurement entry point rkdg_time_steps takes exactly the steps of rkdg_advance
     (graph-replayed pairs after an odd step count, device dt)."""
