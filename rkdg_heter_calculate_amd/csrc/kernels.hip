@@ -324,26 +324,15 @@ struct ElemWork {
     double per;                // perimeter
 };
 
-template <int P, int FLUX, int NBSRC>
-__device__ __forceinline__ void element_residual(const StageArgs& a, int e, const double (&Uo)[4][Deg<P>::Np],
-                                                 const ElemGeo& eg, int w0, int w1, int w2, const double* sVe,
-                                                 const double* sU, ElemWork<P>& W, const double* sGe = nullptr) {
-    using D = Deg<P>;
-    constexpr int Np = D::Np, Nqv = D::Nqv, Nqe = D::Nqe;
-    constexpr bool PF = NBSRC == NB_GLOBAL && Prefetch<P>::on;
+// Volume integral sum_q w_q A (F . grad psi_i), added into R.  The point loop stays rolled:
+// each iteration reads one row of the reference operators into scalar registers instead of
+// keeping every table live.
+template <int P>
+__device__ __forceinline__ void element_volume(const StageArgs& a, const double (&Uo)[4][Deg<P>::Np],
+                                               const ElemGeo& eg, double (&R)[4][Deg<P>::Np]) {
+    constexpr int Np = Deg<P>::Np, Nqv = Deg<P>::Nqv;
     const DevTables& T = c_tab[P - 1];
     const double g = a.gamma;
-    auto& R = W.R;
-    auto& Nb = W.Nb;
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int i = 0; i < Np; ++i) R[v][i] = 0.0;
-    if (PF && w0 >= 0) load_modes<Np>(a.U, w0 >> 2, Nb);
-
-    // ---- volume integral: sum_q w_q A (F . grad psi_i) -----------------------------
-    // The point loop stays rolled: each iteration reads one row of the reference
-    // operators into scalar registers instead of keeping every table live.
 #pragma unroll 1
     for (int q = 0; q < Nqv; ++q) {
         double uq[4];
@@ -366,6 +355,26 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
 #pragma unroll
             for (int i = 1; i < Np; ++i) R[v][i] = fma(Fx[v], T.Dxi[q][i], fma(Fe[v], T.Deta[q][i], R[v][i]));
     }
+}
+
+template <int P, int FLUX, int NBSRC>
+__device__ __forceinline__ void element_residual(const StageArgs& a, int e, const double (&Uo)[4][Deg<P>::Np],
+                                                 const ElemGeo& eg, int w0, int w1, int w2, const double* sVe,
+                                                 const double* sU, ElemWork<P>& W, const double* sGe = nullptr) {
+    using D = Deg<P>;
+    constexpr int Np = D::Np, Nqv = D::Nqv, Nqe = D::Nqe;
+    constexpr bool PF = NBSRC == NB_GLOBAL && Prefetch<P>::on;
+    const DevTables& T = c_tab[P - 1];
+    const double g = a.gamma;
+    auto& R = W.R;
+    auto& Nb = W.Nb;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int i = 0; i < Np; ++i) R[v][i] = 0.0;
+    if (PF && w0 >= 0) load_modes<Np>(a.U, w0 >> 2, Nb);
+
+    element_volume<P>(a, Uo, eg, R);
 
     // ---- surface integral, element-owned -----------------------------------------
     // Both loops stay rolled: one quadrature point's traces and flux are live at a time.
