@@ -137,10 +137,11 @@ def test_run_twice_bitwise():
     assert np.array_equal(a, b)
 
 
-def test_advance_counts_steps_per_call():
-    print("case: advance step counts and end time", flush=True)
+@pytest.mark.parametrize("P", [2, 1])  # P1 LF runs the register-pipelined persistent tiles
+def test_advance_counts_steps_per_call(P):
+    print(f"case: advance step counts and end time P{P}", flush=True)
     g = vortex_grid(16, jitter=0.2, diagonal=1, seed=5)
-    s = Solver(g, degree=2, dt_policy="fixed", dt_fixed=1e-3)
+    s = Solver(g, degree=P, dt_policy="fixed", dt_fixed=1e-3)
     s.project_ic(1, vortex_params())
     n1, t1 = s.advance(5)
     n2, t2 = s.advance(5)
