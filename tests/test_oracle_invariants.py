@@ -4,9 +4,10 @@ Each check is chosen so that a plausible mistake in the restated algorithm (a dr
 term, a wrong sign or index, a transposed operand, a wrong dt subset) fails it:
 quadrature/basis identities (orthogonality under an independent rule, the divergence
 theorem linking derivative, edge and normal tables), free-stream preservation,
-conservation, closed forms of the LF and Roe fluxes on piecewise-constant data, the
-reference's dt subset, SSP-RK3 temporal order, spatial convergence on the exact
-isentropic vortex, and TVB limiter identities.
+conservation (periodic meshes; mass and energy in a box of slip walls), closed forms of the
+LF and Roe fluxes on piecewise-constant data, the Roe flux's consistency, exact stationary
+contact and shock, and antisymmetry, the reference's dt subset, SSP-RK3 temporal order,
+spatial convergence on the exact isentropic vortex, and TVB limiter identities.
 """
 import numpy as np
 import pytest
@@ -340,3 +341,38 @@ def test_positivity_scaling_keeps_point_pressure_positive():
                 worst_after = min(worst_after, m)
     assert worst_before < 0 < worst_after
     assert (o.flags() & 2).sum() > 0 and (o.flags() & 1).sum() == 0
+
+
+def test_roe_flux_is_antisymmetric():
+    """F^(UR, UL, -n) = -F^(UL, UR, n), entropy fix included (the fix's wave-speed bounds map
+    onto each other when the states swap and the normal turns): each interior edge's two
+    element-owned evaluations agree."""
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        def state():
+            r, u, v, p = rng.uniform(0.2, 3), rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.2, 3)
+            return np.array([r, r * u, r * v, p / (G - 1) + 0.5 * r * (u * u + v * v)])
+        UL, UR = state(), state()
+        th = rng.uniform(0, 2 * np.pi)
+        nx, ny = np.cos(th), np.sin(th)
+        a = po.roe_flux(G, UL, UR, nx, ny)
+        b = po.roe_flux(G, UR, UL, -nx, -ny)
+        assert np.allclose(a, -b, rtol=1e-12, atol=1e-12 * np.abs(a).max())
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+def test_reflecting_box_conserves_mass_and_energy(flux):
+    """A closed box of slip walls: the mirrored ghost state (reflection about the edge normal)
+    gives zero mass and energy flux through every wall edge, so the mass and energy residuals
+    sum to zero; the momentum does not (the walls' pressure). A wrong sign in the mirrored
+    velocity makes mass leak."""
+    from rkdg_heter_calculate_amd._lib import BC_REFLECTION
+    g = Grid.splitquad(10, 10, 0.0, 0.0, 10.0, 10.0, periodic=False, diagonal=1, jitter=0.2, seed=3, bc=BC_REFLECTION)
+    o = oracle_for(g, P=2, flux=flux)
+    o.project(1, vortex_params(uinf=0.4, vinf=-0.3, xc=5.0, yc=5.0, lx=0.0, ly=0.0))
+    R = o.residual()
+    tot = R[:, 0, :].sum(axis=1)
+    scale = np.abs(R[:, 0, :]).sum(axis=1)
+    assert abs(tot[0]) < 1e-12 * scale[0], tot
+    assert abs(tot[3]) < 1e-12 * scale[3], tot
+    assert max(abs(tot[1]) / scale[1], abs(tot[2]) / scale[2]) > 1e-3, tot
