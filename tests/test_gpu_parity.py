@@ -424,3 +424,38 @@ def test_precision_form_batch_equals_single(P):
             for v in range(4):
                 assert np.linalg.norm(Ub[v] - Us[v]) <= 2e-6 * np.linalg.norm(Us[v]), (i, v)
     lib.rkdg_batch_destroy(h)
+
+
+class _TinyGrid:
+    """A hand-made mesh: one triangle with a far-field, a slip-wall and a transmissive edge."""
+
+    def __init__(self):
+        self.vx = np.array([0.0, 1.0, 0.2])
+        self.vy = np.array([0.0, 0.1, 0.9])
+        self.tri = np.array([[0, 1, 2]], dtype=np.int32)
+        self.neighbour = np.full((1, 3), -1, dtype=np.int32)
+        self.shared_edge = np.zeros((1, 3), dtype=np.int32)
+        self.boundary = np.array([[BC_FARFIELD, BC_REFLECTION, BC_OUTFLOW]], dtype=np.int32)
+        self.n_elements, self.n_vertices = 1, 3
+
+    def element_coords(self):
+        return self.vx[self.tri], self.vy[self.tri]
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_tiny_meshes_match_oracle(P, prec):
+    """The smallest inputs: one triangle (every edge a different boundary kind) and one split
+    quad (two triangles, far field): a tile of one or two live lanes and 255 padded ones."""
+    print(f"case: tiny meshes P{P} {prec}", flush=True)
+    fs = (1.0, 0.3, -0.2, 2.6)
+    for g in (_TinyGrid(), Grid.splitquad(1, 1, 0.0, 0.0, 1.0, 1.0, periodic=False, diagonal=1, bc=BC_FARFIELD)):
+        o, s = make_pair(g, P=P, flux="roe", dt_policy="max", cfl=0.1, freestream=fs)
+        if prec == "fp32":
+            s = Solver(g, degree=P, flux="roe", dt_policy="max", cfl=0.1, precision="fp32", freestream=list(fs))
+        o.project(1, vortex_params(uinf=0.3, vinf=-0.2, xc=0.4, yc=0.4, lx=0.0, ly=0.0, beta=1.0))
+        s.set_state(o.get_state())
+        s.advance(3)
+        o.advance(3)
+        err = rel_l2(s.get_state(), o.get_state(), o.mass())
+        assert np.all(err < (TOL if prec == "fp64" else MX_TOL)), (g.n_elements, err)

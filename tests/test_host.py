@@ -314,3 +314,15 @@ def test_create_rejects_inconsistent_connectivity(fault):
     assert ei.value.code == L.ERR_MESH
     rc, msg = _batch_create([Grid.splitquad(3, 3, periodic=True), g], degree=1, dt_policy="max", cfl=0.3)
     assert rc == L.ERR_MESH, (rc, msg)
+
+
+def test_create_rejects_an_empty_mesh():
+    """n_elements < 1 is an argument error, reported before any device call."""
+    from rkdg_heter_calculate_amd.solver import make_params
+    lib = L.load()
+    m = L.Mesh()
+    m.n_elements, m.n_vertices = 0, 0
+    prm = make_params(degree=2)
+    h = ctypes.c_void_p()
+    assert lib.rkdg_create(ctypes.byref(m), ctypes.byref(prm), ctypes.byref(h)) == L.ERR_ARG
+    assert lib.rkdg_batch_create(0, None, None, ctypes.byref(h)) == L.ERR_ARG
