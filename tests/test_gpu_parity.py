@@ -459,3 +459,37 @@ def test_tiny_meshes_match_oracle(P, prec):
         o.advance(3)
         err = rel_l2(s.get_state(), o.get_state(), o.mass())
         assert np.all(err < (TOL if prec == "fp64" else MX_TOL)), (g.n_elements, err)
+
+
+def test_non_physical_state_is_reported_with_the_element():
+    """A negative density mean is reported as RKDG_ERR_NONPHYSICAL naming the caller's element
+    (the device position maps back through the locality order); a batch names the instance."""
+    print("case: non-physical state report", flush=True)
+    import ctypes
+    from rkdg_heter_calculate_amd import _lib as L
+    from rkdg_heter_calculate_amd.solver import make_params, mesh_view
+    g = vortex_grid(8, jitter=0.2, diagonal=1, seed=9)
+    s = Solver(g, degree=2, dt_policy="max", cfl=0.3)
+    s.project_ic(1, vortex_params())
+    U = s.get_state()
+    U[0, 0, 37] = -0.5
+    with pytest.raises(L.RkdgError) as ei:
+        s.set_state(U)
+    assert ei.value.code == L.ERR_NONPHYSICAL and "element 37" in str(ei.value), str(ei.value)
+    s.close()
+    lib = L.load()
+    grids = [vortex_grid(6, seed=1), g]
+    views = [mesh_view(x) for x in grids]
+    meshes = (L.Mesh * 2)(*views)
+    params = (L.Params * 2)(*[make_params(degree=2, dt_policy="max", cfl=0.3) for _ in grids])
+    h = ctypes.c_void_p()
+    L.bcheck(lib.rkdg_batch_create(2, meshes, params, ctypes.byref(h)), None, "batch_create")
+    vp = np.array(vortex_params(), dtype=np.float64)
+    L.bcheck(lib.rkdg_batch_project_ic(h, 0, 1, L.dptr(vp), len(vp)), h, "project")
+    Ub = np.ascontiguousarray(U)
+    rc = lib.rkdg_batch_set_state(h, 1, 1, L.dptr(Ub))
+    if rc == 0:
+        rc = lib.rkdg_batch_sync(h)
+    msg = lib.rkdg_batch_last_error(h).decode()
+    lib.rkdg_batch_destroy(h)
+    assert rc == L.ERR_NONPHYSICAL and "instance 1" in msg, (rc, msg)
