@@ -52,7 +52,7 @@ def large_p2_point(n, steps):
     g = workload(n)
     s = Solver(g, degree=2, flux="lf", dt_policy="max", cfl=0.3)
     s.project_ic(IC_VORTEX, vortex_params())
-    s.advance(2)
+    s.advance(40)  # ~40 ms of sustained load first: the clocks ramp up after the host set-up
     ms = s.time_steps(steps)
     B = algorithmic_bytes_per_element_stage(6)
     E = g.n_elements
@@ -87,7 +87,7 @@ def batch_point(count, steps, precision="fp64"):
         L.bcheck(lib.rkdg_batch_project_ic(h, i, 1, L.dptr(vp), len(vp)), h, "batch_project_ic")
     best = ctypes.c_int32()
     L.bcheck(lib.rkdg_batch_autotune(h, ctypes.byref(best)), h, "autotune")
-    L.bcheck(lib.rkdg_batch_advance(h, 2, 1), h, "warmup")
+    L.bcheck(lib.rkdg_batch_advance(h, 10, 1), h, "warmup")  # sustained load before the timed steps
     L.bcheck(lib.rkdg_batch_sync(h), h, "sync")
     t0 = time.perf_counter()
     L.bcheck(lib.rkdg_batch_advance(h, steps, 1), h, "advance")
@@ -113,7 +113,7 @@ def precision_point(g, P, steps):
     from rkdg_heter_calculate_amd import _lib as L
     s = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=0.3, precision="fp32")
     s.project_ic(IC_VORTEX, vortex_params())
-    s.advance(10)
+    s.advance(100)  # sustained load before the timed steps
     s.sync()
     ms = s.time_steps(steps)
     t_sim, _ = s.time()
@@ -242,6 +242,34 @@ def cpu_baseline(g, P, steps, threads):
     return g.n_elements * Np * 4 * 3 * steps / dt, dt
 
 
+def host_cpu():
+    """CPU model, logical CPUs of the host and CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "affinity_cpus": allowed}
+
+
+def cpu_threads_all():
+    """Threads of the all-cores run: the process's CPU share (OMP_NUM_THREADS when the
+    host sets one, otherwise every CPU the process may run on)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    hc = host_cpu()
+    if env.isdigit() and int(env) > 0:
+        return min(int(env), hc["affinity_cpus"])
+    return hc["affinity_cpus"]
+
+
 def read_traffic(name):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
     path = os.path.join(ROOT, "profiles", name)
@@ -264,7 +292,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512, help="split quads per side")
     ap.add_argument("--degree", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=50, help="oracle steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=50, help="oracle steps of the all-cores CPU baseline sample")
+    ap.add_argument("--cpu-steps-1t", type=int, default=3, help="oracle steps of the one-thread CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip every extra leg")
     ap.add_argument("--extra", default="p2_2M,batch",
@@ -293,6 +322,18 @@ def main():
     cfl = 0.3 if P <= 2 else 0.2  # P3: the large-mesh P3 workload's CFL number
     s = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=cfl, precision=args.form)
     s.project_ic(IC_VORTEX, vortex_params())
+    # measured copy bandwidth, right before the warm-up steps: after the host set-up the
+    # GPU has been idle, and its clocks take about 20 ms of sustained load to come back up
+    # (the first stage launches after an idle gap run up to 30% longer), which a few
+    # warm-up steps alone do not cover
+    copy_gbs = None
+    if args.copy_mib > 0:
+        import ctypes
+        from rkdg_heter_calculate_amd import _lib as L
+        gbs = ctypes.c_double()
+        L.check(L.load().rkdg_measure_copy(0, args.copy_mib << 20, 20, ctypes.byref(gbs)), None, "measure_copy")
+        copy_gbs = gbs.value
+        progress(f"stream copy: {copy_gbs:.0f} GB/s")
     progress(f"{E} elements set up ({args.form}); {args.warmup} warm-up steps")
     s.advance(args.warmup)
     s.sync()
@@ -323,20 +364,19 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        hc = host_cpu()
+        threads = cpu_threads_all()
         v, secs = cpu_baseline(g, P, args.cpu_steps, threads)
-        progress(f"cpu baseline: {args.cpu_steps} oracle steps in {secs:.1f} s")
+        progress(f"cpu baseline: {args.cpu_steps} oracle steps on {threads} threads in {secs:.1f} s")
+        v1, secs1 = cpu_baseline(g, P, args.cpu_steps_1t, 1)
+        progress(f"cpu baseline: {args.cpu_steps_1t} oracle steps on 1 thread in {secs1:.1f} s")
         cpu = {"value": v, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} SSP-RK3 steps of the same {E}-triangle P{P} mesh "
-                         f"({secs:.1f} s, OpenMP over elements)"}
-
-    copy_gbs = None
-    if args.copy_mib > 0:
-        import ctypes
-        from rkdg_heter_calculate_amd import _lib as L
-        gbs = ctypes.c_double()
-        L.check(L.load().rkdg_measure_copy(0, args.copy_mib << 20, 20, ctypes.byref(gbs)), None, "measure_copy")
-        copy_gbs = gbs.value
+                         f"({secs:.1f} s, OpenMP over elements, {threads} threads)",
+               "single_thread": {"value": v1, "cores": 1,
+                                 "sample": f"{args.cpu_steps_1t} SSP-RK3 steps of the same mesh ({secs1:.1f} s)"},
+               "cpu_model": hc["cpu_model"], "nproc": hc["nproc"], "affinity_cpus": hc["affinity_cpus"],
+               "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
 
     prec = None
     if not args.no_precision and not fp32:

@@ -183,7 +183,13 @@ const char* rkdg_last_error(rkdg_ctx* ctx);   /* NULL ctx: last global error */
 const char* rkdg_strerror(int32_t code);
 
 /* ---- batching form: many independent meshes in one launch per stage --------------- */
-/* No reference counterpart (the reference runs one mesh per process). */
+/* No reference counterpart (the reference runs one mesh per process; the batch replaces n
+ * runs of RkdgAdvance::advance, src/rkdgadvance.cpp:91-243, one per mesh).
+ * Per instance: the mesh, the initial state and cfl.  Every other rkdg_params field is one
+ * per launch and must agree across the instances (RKDG_ERR_ARG otherwise): degree, flux,
+ * gamma, dt_policy (and dt_fixed under RKDG_DT_FIXED, the reduction_threads subset rule
+ * under RKDG_DT_REFERENCE), precision, limiter (RKDG_LIMITER_NONE only), device and block;
+ * freestream must agree among the instances that have far-field edges. */
 int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* params, rkdg_batch** out);
 int rkdg_batch_destroy(rkdg_batch* b);
 int rkdg_batch_project_ic(rkdg_batch* b, int32_t i, int32_t kind, const double* p, int32_t np);
@@ -194,9 +200,17 @@ int rkdg_batch_get_state(rkdg_batch* b, int32_t i, int32_t basis, double* U);
 int rkdg_batch_advance(rkdg_batch* b, int32_t nsteps, int32_t use_graph);
 int rkdg_batch_time(rkdg_batch* b, int32_t i, double* t);
 int rkdg_batch_sync(rkdg_batch* b);
-/* Autotuner: times candidate workgroup sizes on the batch and keeps the fastest;
- * best_block may be NULL. */
+/* Autotuner: times every tile configuration the batch admits (rkdg_batch_set_tiles) on a
+ * repeatable stage and keeps the fastest; best_block may be NULL. */
 int rkdg_batch_autotune(rkdg_batch* b, int32_t* best_block);
+/* Selects one tile configuration: staged != 0 for the staged 256-element tiles (P1/P2, when
+ * every tile's halo fits its LDS image), else global-gather tiles of block = 64, 128 or 256
+ * elements (the FP32-MFMA form: 64); per_cu 0 launches one workgroup per tile, 4 or 8 a
+ * persistent grid of that many 256-thread workgroups per CU.  RKDG_ERR_ARG when the batch
+ * does not admit it.  rkdg_batch_get_tiles reports the configuration in use as
+ * {staged, block, per_cu}. */
+int rkdg_batch_set_tiles(rkdg_batch* b, int32_t staged, int32_t block, int32_t per_cu);
+int rkdg_batch_get_tiles(rkdg_batch* b, int32_t out[3]);
 int rkdg_batch_info(rkdg_batch* b, int64_t out[4]);   /* instances, total elements, block, device bytes */
 const char* rkdg_batch_last_error(rkdg_batch* b);     /* NULL b: last batch error of this thread */
 

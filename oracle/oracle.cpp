@@ -446,12 +446,52 @@ void vortex_state(const VortexParams& V, double x, double y, double* U) {
     U[0] = rho; U[1] = rho * u; U[2] = rho * v; U[3] = p / (g - 1) + 0.5 * rho * (u * u + v * v);
 }
 
-void riemann4_state(const double* prm, double gamma, double x, double y, double* U) {
-    // prm: xs, ys, then 4 quadrant primitive states (rho,u,v,p): UR, UL, LL, LR
-    int q = (x >= prm[0]) ? (y >= prm[1] ? 0 : 3) : (y >= prm[1] ? 1 : 2);
-    const double* s = prm + 2 + 4 * q;
-    U[0] = s[0]; U[1] = s[0] * s[1]; U[2] = s[0] * s[2];
-    U[3] = s[3] / (gamma - 1) + 0.5 * s[0] * (s[1] * s[1] + s[2] * s[2]);
+// Area of triangle (x, y) inside the box [x0, x1] x [y0, y1]: the triangle clipped by the
+// four box sides in turn (each a half-plane a*x + b*y <= c), then the shoelace formula.
+double triangle_box_area(const double* x, const double* y, double x0, double x1, double y0, double y1) {
+    std::vector<double> px(x, x + 3), py(y, y + 3);
+    const double hp[4][3] = {{-1, 0, -x0}, {1, 0, x1}, {0, -1, -y0}, {0, 1, y1}};
+    for (const auto& h : hp) {
+        std::vector<double> qx, qy;
+        const size_t n = px.size();
+        for (size_t i = 0; i < n; ++i) {
+            const size_t j = (i + 1) % n;
+            const double fi = h[2] - (h[0] * px[i] + h[1] * py[i]);
+            const double fj = h[2] - (h[0] * px[j] + h[1] * py[j]);
+            if (fi >= 0) { qx.push_back(px[i]); qy.push_back(py[i]); }
+            if ((fi >= 0) != (fj >= 0)) {
+                const double t = fi / (fi - fj);
+                qx.push_back(px[i] + t * (px[j] - px[i]));
+                qy.push_back(py[i] + t * (py[j] - py[i]));
+            }
+        }
+        px.swap(qx); py.swap(qy);
+        if (px.size() < 3) return 0.0;
+    }
+    double a = 0;
+    for (size_t i = 0; i < px.size(); ++i) {
+        const size_t j = (i + 1) % px.size();
+        a += px[i] * py[j] - px[j] * py[i];
+    }
+    return 0.5 * std::fabs(a);
+}
+
+// Exact cell average of the four-quadrant data over a triangle; prm: xs, ys, then the four
+// quadrants' primitive states (rho, u, v, p) in the order UR, UL, LL, LR.
+void riemann4_average(const double* prm, double gamma, const double* x, const double* y, double* U) {
+    const double big = 1e300;
+    const double xs = prm[0], ys = prm[1];
+    // quadrants in the state order UR, UL, LL, LR
+    const double box[4][4] = {{xs, big, ys, big}, {-big, xs, ys, big}, {-big, xs, -big, ys}, {xs, big, -big, ys}};
+    double a[4], tot = 0;
+    for (int q = 0; q < 4; ++q) tot += (a[q] = triangle_box_area(x, y, box[q][0], box[q][1], box[q][2], box[q][3]));
+    for (int v = 0; v < 4; ++v) U[v] = 0;
+    for (int q = 0; q < 4; ++q) {
+        const double* s = prm + 2 + 4 * q;
+        const double Uq[4] = {s[0], s[0] * s[1], s[0] * s[2],
+                              s[3] / (gamma - 1) + 0.5 * s[0] * (s[1] * s[1] + s[2] * s[2])};
+        for (int v = 0; v < 4; ++v) U[v] += a[q] / tot * Uq[v];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1045,7 +1085,7 @@ int orc_residual_change(void* h, double* res) {
 // kind 0: free stream of src/cudarkdgsolver.cu:326-368 (prm: gamma, alpha_rad, mach, rhoref, pref)
 // kind 1: isentropic vortex (prm: gamma, beta, uinf, vinf, xc, yc, Lx, Ly, t)
 // kind 2: four-quadrant Riemann data (prm: gamma, xs, ys, 4 x (rho,u,v,p) for UR, UL, LL, LR),
-//         cell averages only
+//         exact cell averages (the triangle clipped to each quadrant)
 int orc_project(void* h, const double* ex, const double* ey, int kind, const double* prm) {
     Solver* S = (Solver*)h;
     const int E = S->E, Np = S->Np;
@@ -1059,6 +1099,17 @@ int orc_project(void* h, const double* ex, const double* ey, int kind, const dou
             S->U[(1 * (size_t)Np) * E + e] = rhoref * u;
             S->U[(2 * (size_t)Np) * E + e] = rhoref * v;
             S->U[(3 * (size_t)Np) * E + e] = rhoref * (ut * ut) / 2 + pref / (rhoref * (g - 1));
+        }
+        return 0;
+    }
+    if (kind == 2) {
+        // discontinuous four-quadrant data start from their exact cell averages (no slopes;
+        // phi_0 = 1, so mode 0 is the mean)
+        std::fill(S->U.begin(), S->U.end(), 0.0);
+        for (int e = 0; e < E; ++e) {
+            double U[4];
+            riemann4_average(prm + 1, prm[0], ex + 3 * e, ey + 3 * e, U);
+            for (int v = 0; v < 4; ++v) S->U[((size_t)v * Np) * E + e] = U[v];
         }
         return 0;
     }
@@ -1077,22 +1128,16 @@ int orc_project(void* h, const double* ex, const double* ey, int kind, const dou
                 double px = x[0] + (x[1] - x[0]) * xi + (x[2] - x[0]) * eta;
                 double py = y[0] + (y[1] - y[0]) * xi + (y[2] - y[0]) * eta;
                 double U[4];
-                if (kind == 1) {
-                    VortexParams V{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7], prm[8]};
-                    vortex_state(V, px, py, U);
-                } else {
-                    riemann4_state(prm + 1, prm[0], px, py, U);
-                }
+                VortexParams V{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7], prm[8]};
+                vortex_state(V, px, py, U);
                 for (int i = 0; i < Np; ++i) {
                     double bf = basis_value(I, i, px, py);
                     for (int v = 0; v < 4; ++v) num[i][v] += w * U[v] * bf;
                     den[i] += w * bf * bf;
                 }
             }
-        // discontinuous four-quadrant data start from cell averages (no slopes)
         for (int i = 0; i < Np; ++i)
-            for (int v = 0; v < 4; ++v)
-                S->U[((size_t)v * Np + i) * E + e] = (kind == 2 && i > 0) ? 0.0 : num[i][v] / den[i];
+            for (int v = 0; v < 4; ++v) S->U[((size_t)v * Np + i) * E + e] = num[i][v] / den[i];
     }
     return 0;
 }

@@ -87,6 +87,8 @@ SIGNATURES = [
     ("rkdg_batch_time", c_int32, [ctypes.c_void_p, c_int32, P_double]),
     ("rkdg_batch_sync", c_int32, [ctypes.c_void_p]),
     ("rkdg_batch_autotune", c_int32, [ctypes.c_void_p, P_int32]),
+    ("rkdg_batch_set_tiles", c_int32, [ctypes.c_void_p, c_int32, c_int32, c_int32]),
+    ("rkdg_batch_get_tiles", c_int32, [ctypes.c_void_p, P_int32]),
     ("rkdg_batch_info", c_int32, [ctypes.c_void_p, P_int64]),
     ("rkdg_batch_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
     ("rkdg_grid_splitquad", c_int32, [c_int32, c_int32, c_double, c_double, c_double, c_double, c_int32, c_int32,

@@ -406,8 +406,72 @@ void ic_state(int kind, const double* p, int np, double x, double y, double* U) 
     }
 }
 
+// Area of a polygon (shoelace) and its clip to the half-plane s * (c - c0) >= 0 of one
+// coordinate (axis 0: x, 1: y; Sutherland-Hodgman, at most n + 1 vertices out).
+static double poly_area(const double* px, const double* py, int n) {
+    double a = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const int j = (i + 1) % n;
+        a += px[i] * py[j] - px[j] * py[i];
+    }
+    return 0.5 * std::fabs(a);
+}
+
+static int clip_half(const double* px, const double* py, int n, int axis, double c0, double s, double* ox,
+                     double* oy) {
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+        const int j = (i + 1) % n;
+        const double ci = s * ((axis ? py[i] : px[i]) - c0), cj = s * ((axis ? py[j] : px[j]) - c0);
+        if (ci >= 0) { ox[m] = px[i]; oy[m] = py[i]; ++m; }
+        if ((ci >= 0) != (cj >= 0)) {
+            const double t = ci / (ci - cj);
+            ox[m] = px[i] + t * (px[j] - px[i]);
+            oy[m] = py[i] + t * (py[j] - py[i]);
+            ++m;
+        }
+    }
+    return m;
+}
+
+// Exact cell average of the four-quadrant data over one triangle: the areas of the
+// triangle's parts in the four quadrants about (xs, ys) (p as ic_state's RKDG_IC_RIEMANN4).
+void riemann4_average(const double* x, const double* y, const double* p, double* U) {
+    double ax[8], ay[8], bx[8], by[8];
+    double area[4] = {0, 0, 0, 0};
+    const double xs = p[1], ys = p[2];
+    for (int sx = 0; sx < 2; ++sx) {  // sx 0: x >= xs, 1: x < xs
+        const int na = clip_half(x, y, 3, 0, xs, sx ? -1.0 : 1.0, ax, ay);
+        for (int sy = 0; sy < 2; ++sy) {  // sy 0: y >= ys, 1: y < ys
+            const int nb = na >= 3 ? clip_half(ax, ay, na, 1, ys, sy ? -1.0 : 1.0, bx, by) : 0;
+            const int q = sx == 0 ? (sy == 0 ? 0 : 3) : (sy == 0 ? 1 : 2);
+            area[q] = nb >= 3 ? poly_area(bx, by, nb) : 0.0;
+        }
+    }
+    const double tot = area[0] + area[1] + area[2] + area[3];
+    for (int v = 0; v < NVAR; ++v) U[v] = 0.0;
+    for (int q = 0; q < 4; ++q) {
+        double Uq[NVAR];
+        const double* s = p + 3 + 4 * q;
+        const double g = p[0];
+        Uq[0] = s[0]; Uq[1] = s[0] * s[1]; Uq[2] = s[0] * s[2];
+        Uq[3] = s[3] / (g - 1) + 0.5 * s[0] * (s[1] * s[1] + s[2] * s[2]);
+        for (int v = 0; v < NVAR; ++v) U[v] += area[q] / tot * Uq[v];
+    }
+}
+
 void project_native(int P, const double* x, const double* y, int kind, const double* p, int np, double* d) {
     const int Np = np_of(P);
+    if (kind == RKDG_IC_RIEMANN4) {
+        // discontinuous four-quadrant data start from their exact cell averages: the L2
+        // projection of a jump onto P_k over- and undershoots and can make the pressure
+        // negative at flux points (psi_0 = 1, so mode 0 is the mean)
+        double U[NVAR];
+        riemann4_average(x, y, p, U);
+        for (int v = 0; v < NVAR; ++v)
+            for (int j = 0; j < Np; ++j) d[v * Np + j] = j == 0 ? U[v] : 0.0;
+        return;
+    }
     const TriRule& R = projection_rule();
     const std::vector<double>& ps = projection_psi(P);
     double num[NVAR][MAXNP] = {}, den[MAXNP] = {};
@@ -422,10 +486,8 @@ void project_native(int P, const double* x, const double* y, int kind, const dou
             den[j] += R.w[q] * s * s;
         }
     }
-    // discontinuous four-quadrant data start from cell averages: the L2 projection of a
-    // jump onto P_k over- and undershoots and can make the pressure negative at flux points
     for (int v = 0; v < NVAR; ++v)
-        for (int j = 0; j < Np; ++j) d[v * Np + j] = (kind == RKDG_IC_RIEMANN4 && j > 0) ? 0.0 : num[v][j] / den[j];
+        for (int j = 0; j < Np; ++j) d[v * Np + j] = num[v][j] / den[j];
 }
 
 // Cockburn-Shu choice of two neighbour barycentres per edge midpoint: m_k - b0 =

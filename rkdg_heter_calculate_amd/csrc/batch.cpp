@@ -155,12 +155,52 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     if (p0.limiter != RKDG_LIMITER_NONE) return bfail(nullptr, RKDG_ERR_ARG, "the batching form runs without limiter");
     if (p0.precision != RKDG_PREC_FP64 && p0.precision != RKDG_PREC_FP32_MFMA)
         return bfail(nullptr, RKDG_ERR_ARG, "precision: RKDG_PREC_FP64 or RKDG_PREC_FP32_MFMA");
-    for (int i = 1; i < n; ++i)
-        if (params[i].degree != p0.degree || params[i].flux != p0.flux || params[i].gamma != p0.gamma ||
-            params[i].dt_policy != p0.dt_policy || params[i].precision != p0.precision)
+    if (p0.block != 0 && (p0.block < 64 || p0.block > 256 || p0.block % 64 != 0 || 256 % p0.block != 0))
+        return bfail(nullptr, RKDG_ERR_ARG, "block: 0 (library default) or 64, 128, 256");
+    // The instances share one launch, so everything but the mesh, the initial state and
+    // the CFL number must agree; a field the batch cannot honour per instance is rejected
+    // rather than taken from instance 0.
+    auto dt_subset = [](const rkdg_params& p) {  // the reference dt subset is on (R >= 128)
+        return p.dt_policy == RKDG_DT_REFERENCE && (p.reduction_threads > 0 ? p.reduction_threads : 512) >= 128;
+    };
+    for (int i = 1; i < n; ++i) {
+        const rkdg_params& pi = params[i];
+        if (pi.degree != p0.degree || pi.flux != p0.flux || pi.gamma != p0.gamma ||
+            pi.dt_policy != p0.dt_policy || pi.precision != p0.precision)
             return bfail(nullptr, RKDG_ERR_ARG, "instances must share degree, flux, gamma, dt policy, precision");
+        if (pi.limiter != p0.limiter || pi.device != p0.device || pi.block != p0.block)
+            return bfail(nullptr, RKDG_ERR_ARG, "instances must share limiter, device and block");
+        if (p0.dt_policy == RKDG_DT_FIXED && pi.dt_fixed != p0.dt_fixed)
+            return bfail(nullptr, RKDG_ERR_ARG, "instances must share dt_fixed");
+        if (dt_subset(pi) != dt_subset(p0))
+            return bfail(nullptr, RKDG_ERR_ARG, "instances must share the dt subset rule (reduction_threads)");
+    }
+    // the far-field state is one per launch: instances with far-field edges must agree on it
+    int ff_src = 0;  // the instance whose free stream the launch uses
+    {
+        int first_ff = -1;
+        for (int i = 0; i < n; ++i) {
+            const rkdg_mesh& m = meshes[i];
+            bool ff = false;
+            if (m.boundary && m.neighbour && m.n_elements > 0)
+                for (long long k = 0; k < 3LL * m.n_elements && !ff; ++k)
+                    ff = m.neighbour[k] < 0 && m.boundary[k] == RKDG_BC_FARFIELD;
+            if (!ff) continue;
+            if (first_ff < 0) { first_ff = i; continue; }
+            for (int v = 0; v < 4; ++v)
+                if (params[i].freestream[v] != params[first_ff].freestream[v])
+                    return bfail(nullptr, RKDG_ERR_ARG,
+                                 "instances with far-field edges must share the free stream (instances " +
+                                     std::to_string(first_ff) + " and " + std::to_string(i) + ")");
+        }
+        if (first_ff > 0) ff_src = first_ff;
+    }
+    for (int i = 0; i < n; ++i)
+        if (meshes[i].n_elements < 1 || !meshes[i].tri || !meshes[i].neighbour || !meshes[i].vx || !meshes[i].vy)
+            return bfail(nullptr, RKDG_ERR_ARG, "mesh " + std::to_string(i));
     auto b = new rkdg_batch();
     b->prm = p0;
+    for (int v = 0; v < 4; ++v) b->prm.freestream[v] = params[ff_src].freestream[v];
     b->n = n;
     b->P = p0.degree;
     b->Np = np_of(b->P);
@@ -175,7 +215,6 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     long long tot = 0;
     for (int i = 0; i < n; ++i) {
         b->E[i] = meshes[i].n_elements;
-        if (b->E[i] < 1 || !meshes[i].tri || !meshes[i].neighbour) { delete b; return bfail(nullptr, RKDG_ERR_ARG, "mesh " + std::to_string(i)); }
         b->off[i] = (int)tot;
         tot += (long long)(b->E[i] + 255) / 256 * 256;
     }
@@ -500,53 +539,97 @@ int rkdg_batch_time(rkdg_batch* b, int32_t i, double* t) {
     return RKDG_OK;
 }
 
+}  // extern "C"
+
+// Workgroups per CU of the device (256 CUs on MI355X).
+static int device_cus(int dev) {
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
+    return cus;
+}
+
+// Checks one tile configuration against the batch's form: staged 256-element tiles (when
+// every halo fits the LDS image), or global-gather tiles of 64, 128 or 256 elements (the
+// precision form's global-gather kernel takes 64-element tiles only); per_cu 0 launches
+// one workgroup per tile, else a persistent grid of per_cu 256-thread workgroups per CU.
+static bool tiles_valid(const rkdg_batch* b, int staged, int block, int per_cu) {
+    if (per_cu != 0 && per_cu != 4 && per_cu != 8) return false;
+    if (staged) return b->staged_ok && b->P <= 2 && block == staged_block(b->P);
+    if (b->prm.precision == RKDG_PREC_FP32_MFMA) return block == 64;
+    return block == 64 || block == 128 || block == 256;
+}
+
+static int apply_tiles(rkdg_batch* b, int staged, int block, int per_cu) {
+    b->staged = staged;
+    b->block = block;
+    b->persistent = per_cu ? device_cus(b->dev) * per_cu * 256 / block : 0;
+    return upload_tiles(b);
+}
+
+extern "C" {
+
+int rkdg_batch_set_tiles(rkdg_batch* b, int32_t staged, int32_t block, int32_t per_cu) {
+    if (!b) return bfail(nullptr, RKDG_ERR_ARG, "batch_set_tiles");
+    if (!tiles_valid(b, staged, block, per_cu))
+        return bfail(b, RKDG_ERR_ARG, "tile configuration not available for this batch");
+    BCHK(b, hipSetDevice(b->dev));
+    return apply_tiles(b, staged ? 1 : 0, block, per_cu);
+}
+
+int rkdg_batch_get_tiles(rkdg_batch* b, int32_t out[3]) {
+    if (!b || !out) return bfail(b, RKDG_ERR_ARG, "batch_get_tiles");
+    const int cus = device_cus(b->dev);
+    out[0] = b->staged;
+    out[1] = b->block;
+    out[2] = b->persistent ? b->persistent * b->block / (256 * cus) : 0;
+    return RKDG_OK;
+}
+
 int rkdg_batch_autotune(rkdg_batch* b, int32_t* best_block) {
     if (!b) return RKDG_ERR_ARG;
     BCHK(b, hipSetDevice(b->dev));
     const int cands[3] = {64, 128, 256};
     const int pers[3] = {0, 4, 8};  // 0: one workgroup per tile; else workgroups per CU
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
     float best = 1e30f;
-    int bb = b->block, bp = b->persistent;
+    int bb = b->block, bs = b->staged;
+    int bp = 0;
+    {
+        int32_t cur[3];
+        rkdg_batch_get_tiles(b, cur);
+        bp = cur[2];
+    }
     hipEvent_t e0, e1;
     BCHK(b, hipEventCreate(&e0));
     BCHK(b, hipEventCreate(&e1));
-    int bs = b->staged;
-    for (int st = 0; st <= (b->staged_ok ? 1 : 0); ++st)
-    for (int c : cands)
-        for (int pw : pers) {
-            if (b->prm.precision == RKDG_PREC_FP32_MFMA && !st && c != 64) continue;
-            if (st && c != staged_block(b->P)) continue;  // the staged form has fixed tiles
-            b->staged = st;
-            b->block = c;
-            b->persistent = pw ? cus * pw * 256 / c : 0;
-            int rc = upload_tiles(b);
-            if (rc) return rc;
-            // stage 1 reads buffers 1 and 0 and writes buffer 2: repeatable without
-            // changing the state
-            StageArgs a = bargs(b, 1, (int)(b->steps & 1));
-            int lrc = launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, a, b->stream);
-            BCHK(b, hipEventRecord(e0, b->stream));
-            for (int r = 0; r < 10 && !lrc; ++r) lrc = launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, a, b->stream);
-            if (lrc) {
-                hipEventDestroy(e0);
-                hipEventDestroy(e1);
-                return bfail(b, RKDG_ERR_HIP, std::string("autotune stage: ") + hip_error_string(lrc));
+    auto finish = [&](int code, const std::string& m) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return bfail(b, code, m);
+    };
+    for (int st = 0; st <= 1; ++st)
+        for (int c : cands)
+            for (int pw : pers) {
+                if (!tiles_valid(b, st, c, pw)) continue;
+                int rc = apply_tiles(b, st, c, pw);
+                if (rc) return finish(rc, b->err);
+                // stage 1 reads buffers 1 and 0 and writes buffer 2: repeatable without
+                // changing the state
+                StageArgs a = bargs(b, 1, (int)(b->steps & 1));
+                int lrc = launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, a, b->stream);
+                hipError_t he = hipEventRecord(e0, b->stream);
+                for (int r = 0; r < 10 && !lrc; ++r) lrc = launch_stage(b->P, b->prm.flux, b->prm.precision, b->block, a, b->stream);
+                if (lrc) return finish(RKDG_ERR_HIP, std::string("autotune stage: ") + hip_error_string(lrc));
+                if (he == hipSuccess) he = hipEventRecord(e1, b->stream);
+                if (he == hipSuccess) he = hipEventSynchronize(e1);
+                float ms = 0;
+                if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+                if (he != hipSuccess) return finish(RKDG_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(he));
+                if (ms < best) { best = ms; bb = c; bp = pw; bs = st; }
             }
-            BCHK(b, hipEventRecord(e1, b->stream));
-            BCHK(b, hipEventSynchronize(e1));
-            float ms;
-            BCHK(b, hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) { best = ms; bb = c; bp = b->persistent; bs = st; }
-        }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    b->block = bb;
-    b->persistent = bp;
-    b->staged = bs;
-    int rc = upload_tiles(b);
+    int rc = apply_tiles(b, bs, bb, bp);
     if (rc) return rc;
     if (best_block) *best_block = bb;
     return RKDG_OK;
