@@ -243,6 +243,16 @@ int rkdg_grid_splitquad(int32_t nx, int32_t ny, double x0, double y0, double lx,
  * markLocalBoundaryTriangles (src/unstructuredgrid.cpp:14-80,193-305,729-760,934-976). */
 int rkdg_grid_read(const char* meshconf, const char* partition_file, double box, rkdg_grid* out);
 
+/* Writes g in the reference mesh format into directory dir (which must exist): mesh.conf
+ * (the keys CUnstructuredGrid::initializeGrid reads, src/unstructuredgrid.cpp:19-36, with
+ * file names dir/<file>), Vertice.dat, ElemVertice.dat, ElemNeigh.dat, ElemEdge.dat,
+ * EdgeVertice.dat, and ElemDepart.dat when g->partition is set — the layouts the readers
+ * of src/unstructuredgrid.cpp:193-305 parse: one header line per file except ElemNeigh
+ * (its header is not skipped, :220) and ElemDepart, 0-based ids, neighbour k across edge k
+ * (-1 on a boundary), edge k opposite vertex k, coordinates to 17 significant digits.
+ * Boundary tags are not part of the format (rkdg_grid_read derives them). */
+int rkdg_grid_write(const char* dir, const rkdg_grid* g);
+
 /* Locality order: by partition id then index (reuse of ElemDepart), or Morton order of
  * barycentres when partition is NULL.  order_out: [n_elements]. */
 int rkdg_grid_locality_order(const rkdg_grid* g, int32_t* order_out);

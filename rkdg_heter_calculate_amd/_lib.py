@@ -94,6 +94,7 @@ SIGNATURES = [
     ("rkdg_grid_splitquad", c_int32, [c_int32, c_int32, c_double, c_double, c_double, c_double, c_int32, c_int32,
                                       c_double, ctypes.c_uint64, c_int32, ctypes.POINTER(Grid)]),
     ("rkdg_grid_read", c_int32, [ctypes.c_char_p, ctypes.c_char_p, c_double, ctypes.POINTER(Grid)]),
+    ("rkdg_grid_write", c_int32, [ctypes.c_char_p, ctypes.POINTER(Grid)]),
     ("rkdg_grid_locality_order", c_int32, [ctypes.POINTER(Grid), P_int32]),
     ("rkdg_grid_free", None, [ctypes.POINTER(Grid)]),
     ("rkdg_grid_last_error", ctypes.c_char_p, []),

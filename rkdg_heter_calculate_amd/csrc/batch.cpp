@@ -243,7 +243,7 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
         b->ey[i].resize((size_t)3 * E);
         std::vector<int32_t> se((size_t)3 * E);
         if (m.shared_edge) std::memcpy(se.data(), m.shared_edge, sizeof(int32_t) * 3 * E);
-        else if (build_shared_edges(E, m.tri, m.neighbour, se.data(), merr)) break;
+        else if (build_shared_edges(E, m.tri, m.neighbour, se.data(), merr, m.vx, m.vy)) break;
         for (int e = 0; e < E; ++e)
             for (int k = 0; k < 3; ++k) {
                 int v = m.tri[3 * e + k];

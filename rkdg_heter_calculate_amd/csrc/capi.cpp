@@ -348,7 +348,7 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
             }
     } else {
         std::string err;
-        int rc = build_shared_edges(E, m->tri, m->neighbour, se.data(), err);
+        int rc = build_shared_edges(E, m->tri, m->neighbour, se.data(), err, m->vx, m->vy);
         if (rc) { delete c; return fail(nullptr, rc, err); }
     }
     // host images of the device arrays (device order)
@@ -602,11 +602,15 @@ int rkdg_compute_dt(rkdg_ctx* c, double* dt) {
     if (c->prm.dt_policy == RKDG_DT_FIXED) { *dt = c->prm.dt_fixed; return RKDG_OK; }
     HIPCHK(c, hipSetDevice(c->dev));
     int slot = (int)(c->steps & 1);
+    // a fresh reduction into this step's wave-speed slot only: the simulated time, the
+    // step count and the dt history stay as they are
+    int rc = launch_ws_clear(c->st, slot, 1, c->stream);
+    if (rc) return fail(c, RKDG_ERR_HIP, hip_error_string(rc));
+    StageArgs a = make_args(c, 0, slot, DTM_DEVICE, 0.0, 0.0);
+    a.U = c->U[c->cur];
+    rc = launch_wavespeed(c->P, a, c->block, c->stream);
+    if (rc) return fail(c, RKDG_ERR_HIP, hip_error_string(rc));
     DevState h;
-    HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    int rc = refresh_wavespeed(c, slot, h.t[slot]);
-    if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     double ws;

@@ -97,7 +97,8 @@ bool read_numbers(const std::string& path, size_t count, std::vector<T>& out) {
 
 }  // namespace
 
-int build_shared_edges(int E, const int32_t* tri, const int32_t* nbr, int32_t* se, std::string& err) {
+int build_shared_edges(int E, const int32_t* tri, const int32_t* nbr, int32_t* se, std::string& err, const double* vx,
+                       const double* vy) {
     for (int e = 0; e < E; ++e)
         for (int k = 0; k < 3; ++k) {
             int n = nbr[3 * e + k];
@@ -109,6 +110,21 @@ int build_shared_edges(int E, const int32_t* tri, const int32_t* nbr, int32_t* s
             for (int kk = 0; kk < 3; ++kk) {
                 int c = tri[3 * n + (kk + 1) % 3], d = tri[3 * n + (kk + 2) % 3];
                 if (c == b && d == a) found = kk;
+            }
+            if (found < 0 && vx && vy) {
+                // a periodic seam: the two sides carry different vertex ids; the shared edge
+                // is the neighbour's edge that points back to this element and is this edge
+                // moved by one translation (its end b maps to the neighbour's start c, a to d)
+                const double lx = vx[b] - vx[a], ly = vy[b] - vy[a];
+                const double tol = 1e-9 * std::sqrt(lx * lx + ly * ly);
+                for (int kk = 0; kk < 3; ++kk) {
+                    if (nbr[3 * n + kk] != e) continue;
+                    int c = tri[3 * n + (kk + 1) % 3], d = tri[3 * n + (kk + 2) % 3];
+                    const double tx = vx[c] - vx[b], ty = vy[c] - vy[b];
+                    if (std::fabs(vx[d] - vx[a] - tx) <= tol && std::fabs(vy[d] - vy[a] - ty) <= tol &&
+                        std::fabs(vx[d] - vx[c] + lx) <= tol && std::fabs(vy[d] - vy[c] + ly) <= tol)
+                        found = kk;
+                }
             }
             if (found < 0) {
                 char buf[160];
@@ -221,7 +237,7 @@ int rkdg_grid_read(const char* meshconf, const char* partition_file, double box,
     for (int i = 0; i < 3 * E; ++i)
         if (tri[i] < 0 || tri[i] >= NV || nbr[i] < -1 || nbr[i] >= E) { g_grid_err = "id out of range"; return RKDG_ERR_MESH; }
     std::vector<int32_t> se((size_t)3 * E), bnd((size_t)3 * E, 0);
-    int rc = build_shared_edges(E, tri.data(), nbr.data(), se.data(), g_grid_err);
+    int rc = build_shared_edges(E, tri.data(), nbr.data(), se.data(), g_grid_err, vx.data(), vy.data());
     if (rc) return rc;
     // boundary tag from the barycentre of the mirrored (ghost) triangle, as
     // markLocalBoundaryTriangles (src/unstructuredgrid.cpp:729-760)
@@ -247,6 +263,88 @@ int rkdg_grid_read(const char* meshconf, const char* partition_file, double box,
     out->vx = dup(vx); out->vy = dup(vy);
     out->tri = dup(tri); out->neighbour = dup(nbr); out->shared_edge = dup(se); out->boundary = dup(bnd);
     out->partition = part.empty() ? nullptr : dup(part);
+    return RKDG_OK;
+}
+
+int rkdg_grid_write(const char* dir, const rkdg_grid* g) {
+    if (!dir || !g || g->n_elements < 1 || g->n_vertices < 1 || !g->vx || !g->vy || !g->tri || !g->neighbour) {
+        g_grid_err = "grid_write arguments";
+        return RKDG_ERR_ARG;
+    }
+    const int E = g->n_elements, NV = g->n_vertices;
+    // edge k of element e lies opposite vertex k; an edge takes its id where it is first met
+    // (ascending element, then edge), and the neighbour across it reuses that id
+    std::vector<int32_t> eid((size_t)3 * E, -1);
+    std::vector<int32_t> ev;
+    std::vector<int32_t> se((size_t)3 * E);
+    if (g->shared_edge) std::memcpy(se.data(), g->shared_edge, sizeof(int32_t) * 3 * E);
+    else if (int rc = build_shared_edges(E, g->tri, g->neighbour, se.data(), g_grid_err, g->vx, g->vy)) return rc;
+    for (int e = 0; e < E; ++e)
+        for (int k = 0; k < 3; ++k) {
+            const int n = g->neighbour[3 * e + k];
+            if (n >= E || (n >= 0 && (se[3 * e + k] < 0 || se[3 * e + k] > 2))) {
+                g_grid_err = "neighbour or shared edge out of range";
+                return RKDG_ERR_MESH;
+            }
+            if (n >= 0 && n < e) {
+                eid[3 * e + k] = eid[3 * n + se[3 * e + k]];
+                continue;
+            }
+            eid[3 * e + k] = (int32_t)(ev.size() / 2);
+            ev.push_back(g->tri[3 * e + (k + 1) % 3]);
+            ev.push_back(g->tri[3 * e + (k + 2) % 3]);
+        }
+    const int NE = (int)(ev.size() / 2);
+    const std::string d(dir);
+    auto path = [&](const char* f) { return d + "/" + f; };
+    auto open = [&](const std::string& p, std::FILE*& f) {
+        f = std::fopen(p.c_str(), "w");
+        if (!f) g_grid_err = "cannot write " + p;
+        return f != nullptr;
+    };
+    std::FILE* f = nullptr;
+    // one header line each (the reference's readers skip it), except ElemNeigh, whose
+    // reader does not (src/unstructuredgrid.cpp:220)
+    if (!open(path("Vertice.dat"), f)) return RKDG_ERR_IO;
+    std::fprintf(f, "#vertist data\n");
+    for (int i = 0; i < NV; ++i) std::fprintf(f, "%.17g %.17g\n", g->vx[i], g->vy[i]);
+    bool ok = std::fclose(f) == 0;
+    if (!open(path("ElemVertice.dat"), f)) return RKDG_ERR_IO;
+    std::fprintf(f, "#ElemVertex Data\n");
+    for (int e = 0; e < E; ++e) std::fprintf(f, "%d %d %d\n", g->tri[3 * e], g->tri[3 * e + 1], g->tri[3 * e + 2]);
+    ok = std::fclose(f) == 0 && ok;
+    if (!open(path("ElemNeigh.dat"), f)) return RKDG_ERR_IO;
+    for (int e = 0; e < E; ++e)
+        std::fprintf(f, "%d %d %d\n", g->neighbour[3 * e], g->neighbour[3 * e + 1], g->neighbour[3 * e + 2]);
+    ok = std::fclose(f) == 0 && ok;
+    if (!open(path("ElemEdge.dat"), f)) return RKDG_ERR_IO;
+    std::fprintf(f, "#Elemside data\n");
+    for (int e = 0; e < E; ++e) std::fprintf(f, "%d %d %d\n", eid[3 * e], eid[3 * e + 1], eid[3 * e + 2]);
+    ok = std::fclose(f) == 0 && ok;
+    if (!open(path("EdgeVertice.dat"), f)) return RKDG_ERR_IO;
+    std::fprintf(f, "#Sidelist data\n");
+    for (int i = 0; i < NE; ++i) std::fprintf(f, "%d %d\n", ev[2 * i], ev[2 * i + 1]);
+    ok = std::fclose(f) == 0 && ok;
+    if (g->partition) {
+        // gridpartition's output: one part id per line, no header (src/unstructuredgrid.cpp:290-305)
+        if (!open(path("ElemDepart.dat"), f)) return RKDG_ERR_IO;
+        for (int e = 0; e < E; ++e) std::fprintf(f, "%d\n", g->partition[e]);
+        ok = std::fclose(f) == 0 && ok;
+    }
+    // mesh.conf: the keys of CUnstructuredGrid::initializeGrid (src/unstructuredgrid.cpp:19-36);
+    // file names as written here (relative to the working directory when dir is relative,
+    // like the reference's input/ paths)
+    if (!open(path("mesh.conf"), f)) return RKDG_ERR_IO;
+    std::fprintf(f, "# mesh written by rkdg_grid_write\n");
+    std::fprintf(f, "trianglenum %d\nverticenum %d\nedgenum %d\n", E, NV, NE);
+    std::fprintf(f, "trivertice_filename %s\n", path("ElemVertice.dat").c_str());
+    std::fprintf(f, "trineigh_filename %s\n", path("ElemNeigh.dat").c_str());
+    std::fprintf(f, "triedge_filename %s\n", path("ElemEdge.dat").c_str());
+    std::fprintf(f, "vertice_filename %s\n", path("Vertice.dat").c_str());
+    std::fprintf(f, "edgevertice_filename %s\n", path("EdgeVertice.dat").c_str());
+    std::fprintf(f, "output_filename %s\n", path("mesh.plt").c_str());
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok) { g_grid_err = "write error in " + d; return RKDG_ERR_IO; }
     return RKDG_OK;
 }
 

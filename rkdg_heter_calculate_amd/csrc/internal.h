@@ -129,7 +129,10 @@ void limiter_geometry(const double* x, const double* y, const double nbx[3], con
                       double* lim, int* pairs);
 
 // Mesh helpers (grid.cpp)
-int build_shared_edges(int E, const int32_t* tri, const int32_t* nbr, int32_t* se, std::string& err);
+// se[3e+k]: the neighbour's local index of edge k of element e, matched by vertex ids, or
+// (with vertex coordinates) across a periodic seam by the translated edge that points back.
+int build_shared_edges(int E, const int32_t* tri, const int32_t* nbr, int32_t* se, std::string& err,
+                       const double* vx = nullptr, const double* vy = nullptr);
 // Morton (Z-order) permutation of points (bx, by): order[p] = index of the p-th point.
 void morton_order(int n, const double* bx, const double* by, int32_t* order);
 // Staged-form tables for tiles of B consecutive device positions: per tile the sorted
@@ -217,6 +220,7 @@ int upload_tables(int P, const DevTables& t);  // per current device
 int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream);
 int launch_wavespeed(int P, const StageArgs& a, int block, void* stream);
 int launch_state_init(DevState* st, int slot, double t, int n, void* stream);
+int launch_ws_clear(DevState* st, int slot, int n, void* stream);
 int launch_limiter(int P, const LimArgs& a, int block, void* stream);
 int launch_stream_copy(const void* src, void* dst, long long bytes, void* stream);
 const char* hip_error_string(int code);

@@ -1032,6 +1032,13 @@ __global__ void __launch_bounds__(256) k_wavespeed(StageArgs a) {
     }
 }
 
+// Clears only the wave-speed maximum of one step parity (a fresh CFL reduction of the
+// current state), leaving the times, dts, step count and error position as they are.
+__global__ void k_ws_clear(DevState* st, int n, int slot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) st[i].ws[slot] = 0ull;
+}
+
 __global__ void k_state_init(DevState* st, int n, int slot, double t) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -1310,6 +1317,11 @@ int launch_wavespeed(int P, const StageArgs& a, int block, void* stream) {
         case 3: hipLaunchKernelGGL(k_wavespeed<3>, grid, blk, 0, s, a); break;
         default: return (int)hipErrorInvalidValue;
     }
+    return (int)hipGetLastError();
+}
+
+int launch_ws_clear(DevState* st, int slot, int n, void* stream) {
+    hipLaunchKernelGGL(k_ws_clear, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, st, n, slot);
     return (int)hipGetLastError();
 }
 

@@ -65,6 +65,34 @@ class Grid:
             self._lib.rkdg_grid_free(ctypes.byref(g))
             self._g = None
 
+    def write(self, directory):
+        """Reference mesh format (mesh.conf + *.dat) into an existing directory; the arrays
+        written are this object's (edits to them are written)."""
+        g = self._sync()
+        rc = self._lib.rkdg_grid_write(str(directory).encode(), ctypes.byref(g))
+        if rc != L.OK:
+            raise L.RkdgError(rc, self._lib.rkdg_grid_last_error().decode())
+
+    def _sync(self):
+        """An rkdg_grid view of the numpy arrays held here."""
+        g = L.Grid()
+        self._wkeep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._wkeep.append(a)
+            return a
+
+        g.n_vertices, g.n_elements = self.n_vertices, self.n_elements
+        g.vx = L.dptr(arr(self.vx, np.float64))
+        g.vy = L.dptr(arr(self.vy, np.float64))
+        g.tri = L.iptr(arr(self.tri, np.int32))
+        g.neighbour = L.iptr(arr(self.neighbour, np.int32))
+        g.shared_edge = L.iptr(arr(self.shared_edge, np.int32))
+        g.boundary = L.iptr(arr(self.boundary, np.int32))
+        g.partition = L.iptr(arr(self.partition, np.int32)) if self.partition is not None else None
+        return g
+
     def element_coords(self):
         return self.vx[self.tri], self.vy[self.tri]
 
@@ -263,3 +291,81 @@ class Solver:
         out = np.zeros(5, np.int64)
         self._chk(self.lib.rkdg_info(self.h, out.ctypes.data_as(L.P_int64)), "info")
         return {"Np": int(out[0]), "Nqv": int(out[1]), "Nqe": int(out[2]), "E": int(out[3]), "bytes": int(out[4])}
+
+
+class Batch:
+    """Many independent meshes advanced together, one launch per stage (rkdg_batch_*).
+
+    params: one keyword dict per instance (make_params' keywords); instances share every
+    field but the mesh, the initial state and cfl (rkdg_batch_create rejects the rest)."""
+
+    def __init__(self, grids, params):
+        self.lib = L.load()
+        n = len(grids)
+        if n != len(params):
+            raise ValueError("one parameter set per mesh")
+        self.grids = grids
+        self._views = [mesh_view(g) for g in grids]
+        meshes = (L.Mesh * n)(*self._views)
+        prm = (L.Params * n)(*[make_params(**p) for p in params])
+        self.degree = prm[0].degree
+        self.Np = (self.degree + 1) * (self.degree + 2) // 2
+        h = ctypes.c_void_p()
+        rc = self.lib.rkdg_batch_create(n, meshes, prm, ctypes.byref(h))
+        if rc != L.OK:
+            raise L.RkdgError(rc, self.lib.rkdg_batch_last_error(None).decode())
+        self.h = h
+        self.n = n
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rkdg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        return L.bcheck(rc, self.h, what)
+
+    def project_ic(self, i, kind, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        self._chk(self.lib.rkdg_batch_project_ic(self.h, i, kind, L.dptr(p), len(p)), "batch_project_ic")
+
+    def set_state(self, i, U, basis=L.BASIS_ELEMENT):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(4, self.Np, self.grids[i].n_elements)
+        self._chk(self.lib.rkdg_batch_set_state(self.h, i, basis, L.dptr(U)), "batch_set_state")
+
+    def get_state(self, i, basis=L.BASIS_ELEMENT):
+        U = np.empty((4, self.Np, self.grids[i].n_elements), np.float64)
+        self._chk(self.lib.rkdg_batch_get_state(self.h, i, basis, L.dptr(U)), "batch_get_state")
+        return U
+
+    def advance(self, nsteps, use_graph=True):
+        self._chk(self.lib.rkdg_batch_advance(self.h, nsteps, int(use_graph)), "batch_advance")
+
+    def sync(self):
+        self._chk(self.lib.rkdg_batch_sync(self.h), "batch_sync")
+
+    def time(self, i):
+        t = ctypes.c_double()
+        self._chk(self.lib.rkdg_batch_time(self.h, i, ctypes.byref(t)), "batch_time")
+        return t.value
+
+    def autotune(self):
+        best = ctypes.c_int32()
+        self._chk(self.lib.rkdg_batch_autotune(self.h, ctypes.byref(best)), "batch_autotune")
+        return best.value
+
+    def set_tiles(self, staged, block, per_cu):
+        self._chk(self.lib.rkdg_batch_set_tiles(self.h, int(staged), int(block), int(per_cu)), "batch_set_tiles")
+
+    def tiles(self):
+        out = np.zeros(3, np.int32)
+        self._chk(self.lib.rkdg_batch_get_tiles(self.h, L.iptr(out)), "batch_get_tiles")
+        return {"staged": int(out[0]), "block": int(out[1]), "per_cu": int(out[2])}
+
+    def info(self):
+        out = np.zeros(4, np.int64)
+        self._chk(self.lib.rkdg_batch_info(self.h, out.ctypes.data_as(L.P_int64)), "batch_info")
+        return {"instances": int(out[0]), "elements": int(out[1]), "block": int(out[2]), "bytes": int(out[3])}

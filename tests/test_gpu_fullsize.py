@@ -1,0 +1,98 @@
+"""GPU against the oracle at the workloads' full sizes (the oracle takes seconds there).
+
+* the 512x512 jittered vortex mesh of the P2 benchmark (seed 2024, 524,288 triangles,
+  2,048 staged tiles, XCD-remapped sweep, halo capacity checked per tile), 5 steps, in
+  the FP64 form (1e-12) and the FP32-MFMA form (1e-6);
+* the 1024x1024 four-quadrant Riemann mesh (seed 2025, 2,097,152 triangles), P1, Roe,
+  TVB M = 50 with positivity scaling, 3 steps (1e-12, equal limiter flags);
+* the same problem shrunk to 64x64 and run to t = 0.3 (about 3,500 steps), where the
+  limiter's decisions could drift apart: relative L2 and the fraction of equal flags.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from rkdg_heter_calculate_amd import IC_RIEMANN4, IC_VORTEX, Solver, vortex_params
+from rkdg_heter_calculate_amd.workloads import riemann_p1, vortex_grid
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+_REF = {}
+
+
+def oracle_for(g, **kw):
+    ex, ey = g.element_coords()
+    return po.Oracle(ex, ey, g.neighbour, g.shared_edge, g.boundary, **kw)
+
+
+def vortex_512_reference():
+    if "v512" not in _REF:
+        g = vortex_grid(512)  # seed 2024, jitter 0.2, seed-chosen diagonals
+        o = oracle_for(g, P=2, flux=0, dt_policy=1, cfl=0.3)
+        o.project(1, vortex_params())
+        U0 = o.get_state()
+        n, t, _ = o.advance(5)
+        _REF["v512"] = (g, U0, o.get_state(), t, o.mass())
+    return _REF["v512"]
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
+def test_vortex_p2_512_matches_oracle(prec, tol):
+    print(f"case: 512x512 P2 vortex 5 steps {prec}", flush=True)
+    g, U0, Uo, to, M = vortex_512_reference()
+    assert g.n_elements == 524288
+    s = Solver(g, degree=2, flux="lf", dt_policy="max", cfl=0.3, precision=prec)
+    s.set_state(U0)
+    n, t = s.advance(5)
+    assert n == 5 and abs(t - to) <= 1e-13 * to
+    Us = s.get_state()
+    err = rel_l2(Us, Uo, M)
+    linf = [float(np.max(np.abs(Us[v] - Uo[v])) / np.max(np.abs(Uo[v]))) for v in range(4)]
+    print("relative L2 per variable", err, "relative Linf", linf, flush=True)
+    assert np.all(err < tol), err
+    s.close()
+
+
+def test_riemann_p1_1024_three_steps_match_oracle():
+    print("case: 1024x1024 four-quadrant Riemann P1 Roe TVB 3 steps", flush=True)
+    g, kw, ic, _ = riemann_p1(1024)
+    assert g.n_elements == 2097152
+    o = oracle_for(g, P=1, flux=1, limiter=1, dt_policy=1, cfl=kw["cfl"], tvb_M=kw["tvb_M"])
+    o.project(2, ic)
+    s = Solver(g, **kw)
+    s.project_ic(IC_RIEMANN4, ic)
+    assert np.array_equal(s.get_state(), o.get_state())  # exact cell averages on both sides
+    s.advance(3)
+    o.advance(3)
+    err = rel_l2(s.get_state(), o.get_state(), o.mass())
+    fs, fo = s.flags(), o.flags()
+    print("relative L2 per variable", err, "limited cells", int((fo != 0).sum()), flush=True)
+    assert np.all(err < 1e-12), err
+    assert np.array_equal(fs, fo)
+    assert (fo != 0).sum() > 1000  # the limiter is active along the four discontinuities
+
+
+def test_riemann_p1_64_to_t03_against_oracle():
+    print("case: 64x64 four-quadrant Riemann P1 Roe TVB to t = 0.3", flush=True)
+    g, kw, ic, t_end = riemann_p1(64)
+    o = oracle_for(g, P=1, flux=1, limiter=1, dt_policy=1, cfl=kw["cfl"], tvb_M=kw["tvb_M"])
+    o.project(2, ic)
+    s = Solver(g, **kw)
+    s.project_ic(IC_RIEMANN4, ic)
+    n = 0
+    t = 0.0
+    while t < t_end:
+        k, t = s.advance(5000, t_end=t_end)
+        n += k
+        if k == 0:
+            break
+    no, to, _ = o.advance(1000000, t_end=t_end)
+    assert n == no and abs(t - to) <= 1e-12
+    err = rel_l2(s.get_state(), o.get_state(), o.mass())
+    fs, fo = s.flags(), o.flags()
+    agree = float(np.mean(fs == fo))
+    print(f"{n} steps to t = {t}: relative L2 per variable {err}, flags agree on {agree:.6f} of the cells "
+          f"({int((fo != 0).sum())} limited)", flush=True)
+    assert np.all(err < 1e-9), err
+    assert agree > 0.999, agree

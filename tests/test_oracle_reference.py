@@ -108,3 +108,47 @@ def test_config_golden_values_are_plain_data():
     c = dict(zip(d["constructed"]["keys"], d["constructed"]["values"]))
     assert c == {"gamma": "1.4", "cfl": "0.25", "mach": "0.8", "title": "my", "pref": "", "rhoref": "2.5",
                  "time": "3"}
+
+
+def test_reference_reader_reads_the_written_mesh(tmp_path):
+    """The reference's own grid set-up (CUnstructuredGrid, run through oracle/_ref) reads a
+    mesh written by rkdg_grid_write and builds the same neighbours, shared edges, ghost
+    flags and ghost barycentres as from its original NACA0012 files."""
+    ref = po.load_ref()
+    if ref is None:
+        pytest.skip("reference build (oracle/_ref) absent")
+    from rkdg_heter_calculate_amd import Grid
+    from test_host import write_reference_mesh
+    d = np.load(os.path.join(GOLD, "naca_grid.npz"))
+    src = tmp_path / "src"
+    src.mkdir()
+    conf, part = write_reference_mesh(src, d)
+    g = Grid.read(conf, part, box=15.0)
+    # one rank: every element in part 0 (with parts beyond 0 and one rank the reference reads
+    # a zero-length array, src/rkdgadvance.cpp:55 and src/unstructuredgrid.cpp:57-59)
+    g.partition = np.zeros(g.n_elements, np.int32)
+    run = tmp_path / "run"
+    (run / "input").mkdir(parents=True)
+    g.write(run / "input")  # the reference reads input/ElemDepart.dat from its working directory
+    counts = np.zeros(2, np.int32)
+    rc = ref.ref_grid(str(run).encode(), str(run / "input" / "mesh.conf").encode(), counts.ctypes.data_as(po.I),
+                      None, None, None, None)
+    assert rc == 0
+    cn = int(counts[1])
+    assert int(counts[0]) == int(d["ref_counts"][0]) and cn == d["ref_flag"].shape[0]
+    nb = np.zeros(3 * cn, np.int32)
+    se = np.zeros(3 * cn, np.int32)
+    fl = np.zeros(cn, np.int32)
+    ba = np.zeros(2 * cn)
+    rc = ref.ref_grid(str(run).encode(), str(run / "input" / "mesh.conf").encode(), counts.ctypes.data_as(po.I),
+                      nb.ctypes.data_as(po.I), se.ctypes.data_as(po.I), fl.ctypes.data_as(po.I),
+                      ba.ctypes.data_as(po.D))
+    assert rc == 0
+    assert np.array_equal(nb.reshape(3, cn), d["ref_neighbour"])
+    # the reference sets a ghost cell's shared edge 0 only (rows 1, 2 stay uninitialised)
+    E = int(counts[0])
+    se = se.reshape(3, cn)
+    assert np.array_equal(se[:, :E], d["ref_shared_edge"][:, :E])
+    assert np.array_equal(se[0, E:], d["ref_shared_edge"][0, E:])
+    assert np.array_equal(fl, d["ref_flag"])
+    assert np.allclose(ba.reshape(2, cn), d["ref_bary"], rtol=0, atol=1e-12)
