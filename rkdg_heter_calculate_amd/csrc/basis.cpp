@@ -152,14 +152,15 @@ void build_ref_basis(int P, RefBasis& B) {
         T.recon[0][k] = (a22 * A[k][0] - a12 * A[k][1]) / det;
         T.recon[1][k] = (a11 * A[k][1] - a12 * A[k][0]) / det;
     }
-    // single-precision operators: rows = evaluation points (volume then edges)
-    for (int q = 0; q < nq; ++q)
-        for (int i = 0; i < Np; ++i) T.fVv[q][i] = (float)T.Vv[q][i];
-    for (int k = 0; k < 3; ++k)
-        for (int j = 0; j < ne; ++j)
-            for (int i = 0; i < Np; ++i) T.fVv[nq + k * ne + j][i] = (float)T.Ve[k][j][i];
-    for (int i = 0; i < Np; ++i)
-        for (int p = 0; p < 16; ++p) T.fTest[i][p] = T.fVv[p][i];
+    // single-precision operators of the P1/P2 precision kernels: rows = evaluation points
+    // (volume then edges: 7 + 3 x 3 = 16; P3's 12 + 3 x 4 points have their own tables)
+    if (nq + 3 * ne <= 16) {
+        for (int q = 0; q < nq; ++q)
+            for (int i = 0; i < Np; ++i) T.fVv[q][i] = (float)T.Vv[q][i];
+        for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < ne; ++j)
+                for (int i = 0; i < Np; ++i) T.fVv[nq + k * ne + j][i] = (float)T.Ve[k][j][i];
+    }
     for (int q = 0; q < nq; ++q) {
         T.fwv[q] = (float)T.wv[q];
         for (int i = 0; i < Np; ++i) {

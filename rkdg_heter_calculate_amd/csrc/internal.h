@@ -64,8 +64,7 @@ struct DevTables {
     double Vmid[3][3];             // P1 modes at the edge midpoints (limiter)
     double recon[2][3];            // modes 1,2 from midpoint deviations (limiter)
     // single-precision copies for the MFMA form
-    float fVv[16][16];             // [point (vol 0..Nqv-1, then 3*Nqe edge)][mode], zero padded
-    float fTest[16][16];           // [mode][point] test functions (gradients handled separately)
+    float fVv[16][16];             // P1/P2: [point (vol 0..Nqv-1, then 3*Nqe edge)][mode], zero padded
     float fDxi[MAXQV][MAXNP];      // single-precision gradients at the volume points
     float fDeta[MAXQV][MAXNP];
     float fwv[MAXQV];
