@@ -102,16 +102,16 @@ def batch_point(count, steps, precision="fp64"):
             "note": f"{precision} batched form, hipGraph-replayed stages, autotuned tiles and persistence"}
 
 
-def precision_point(g, P, steps):
-    """FP32-MFMA form on the same workload: timing and its error against the FP64 form.
-
-    The error runs use one fixed dt (the FP64 form's first CFL dt) so that both forms take
-    identical steps; the FP64 form matches the oracle to 1e-12, so the difference is the
-    precision form's own error."""
+def precision_point(g, P, steps, prec="fp32"):
+    """A precision form (FP32 or BF16 contractions, FP64 state) on the same workload: its
+    timing; then the FP64 form and it run the same `steps` steps of one fixed dt, and give
+    each form's error against the exact vortex (their ratio: "matched" when within 10%) and
+    the form's difference to the FP64 form (which matches the oracle to 1e-12, so that
+    difference is the form's own error)."""
     import numpy as np
     from rkdg_heter_calculate_amd import Solver, IC_VORTEX, vortex_params
     from rkdg_heter_calculate_amd import _lib as L
-    s = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=0.3, precision="fp32")
+    s = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=0.3, precision=prec)
     s.project_ic(IC_VORTEX, vortex_params())
     s.advance(100)  # sustained load before the timed steps
     s.sync()
@@ -122,28 +122,34 @@ def precision_point(g, P, steps):
     E = g.n_elements
     Np = (P + 1) * (P + 2) // 2
     out = {"value": E * Np * 4 * 3 * steps / (ms / 1e3), "ms_per_step": ms / steps,
-           "stage_ms": ms / (3 * steps), "l2_vs_exact": err_exact["l2"], "linf_vs_exact": err_exact["linf"]}
+           "stage_ms": ms / (3 * steps), "l2_vs_exact_timed_run": err_exact["l2"]}
     # matched-step comparison against the FP64 form
     d = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=0.3)
     d.project_ic(IC_VORTEX, vortex_params())
     dt0 = d.compute_dt()
     d.close()
-    res = {}
-    for prec in ("fp64", "fp32"):
-        x = Solver(g, degree=P, flux="lf", dt_policy="fixed", dt_fixed=dt0, precision=prec)
+    res, exact = {}, {}
+    for form in ("fp64", prec):
+        x = Solver(g, degree=P, flux="lf", dt_policy="fixed", dt_fixed=dt0, precision=form)
         x.project_ic(IC_VORTEX, vortex_params())
         x.advance(steps)
-        res[prec] = x.get_state(L.BASIS_NATIVE)  # orthonormal on the reference triangle
+        t_f, _ = x.time()
+        exact[form] = x.error_vs_exact(IC_VORTEX, vortex_params(t=t_f))
+        res[form] = x.get_state(L.BASIS_NATIVE)  # orthonormal on the reference triangle
         x.close()
-    a, b = res["fp64"], res["fp32"]
-    ex, ey = g.element_coords()
-    area = 0.5 * np.abs((ex[:, 1] - ex[:, 0]) * (ey[:, 2] - ey[:, 0]) - (ex[:, 2] - ex[:, 0]) * (ey[:, 1] - ey[:, 0]))
+    ratio = [a / b for a, b in zip(exact[prec]["l2"], exact["fp64"]["l2"])]
+    out.update({"l2_vs_exact": exact[prec]["l2"], "linf_vs_exact": exact[prec]["linf"],
+                "l2_vs_exact_fp64_same_steps": exact["fp64"]["l2"], "l2_vs_exact_ratio_to_fp64": ratio,
+                "matched_l2": bool(all(abs(r - 1.0) <= 0.1 for r in ratio))})
+    a, b = res["fp64"], res[prec]
+    area = _areas(g)
     # L2 norm of a field = sqrt(sum_e |J_e| sum_i c_i^2) in the orthonormal reference basis
-    rel = [float(np.sqrt(np.sum(area * np.sum((b[v] - a[v]) ** 2, axis=0)) / np.sum(area * np.sum(a[v] ** 2, axis=0))))
-           for v in range(4)]
+    rel = _rel_l2(a, b, area)
     linf = [float(np.max(np.abs(b[v] - a[v])) / np.max(np.abs(a[v]))) for v in range(4)]
+    kind = "FP32 on v_mfma_f32_4x4x1f32" if prec == "fp32" else "BF16 on v_mfma_f32_4x4x4_16b_bf16"
     out.update({"rel_l2_vs_fp64": rel, "rel_linf_vs_fp64": linf, "compare_steps": steps, "compare_dt": dt0,
-                "note": "FP32 contractions on v_mfma_f32_4x4x1f32, FP64 state, means, alpha, RK and CFL"})
+                "note": f"{kind} contractions of the deviations from the FP64 cell mean (FP32 accumulation); "
+                        "FP64 state, means, fluxes, alpha, RK and CFL"})
     return out
 
 
@@ -194,8 +200,9 @@ def riemann_point(n=1024, t_end=0.3):
 
 
 def p3_point(n=2048, steps=50):
-    """Large P3 vortex (Roe, CFL 0.2): FP64 and the precision form, stage-kernel roofline and
-    the precision form's difference after the same fixed-dt steps."""
+    """Large P3 vortex (Roe, CFL 0.2): FP64 and the precision form with FP32 and with BF16
+    contractions, stage-kernel roofline and each precision form's difference to FP64 after
+    the same fixed-dt steps."""
     from rkdg_heter_calculate_amd import Solver, IC_VORTEX, vortex_params
     from rkdg_heter_calculate_amd import _lib as L
     from rkdg_heter_calculate_amd.workloads import vortex_p3
@@ -205,7 +212,7 @@ def p3_point(n=2048, steps=50):
     B = algorithmic_bytes_per_element_stage(10)
     dt0 = None
     states = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16"):
         s = Solver(g, precision=prec, **kw)
         s.project_ic(IC_VORTEX, vp)
         if dt0 is None:
@@ -222,7 +229,9 @@ def p3_point(n=2048, steps=50):
         f.advance(steps)
         states[prec] = f.get_state(L.BASIS_NATIVE)
         f.close()
-    out["rel_l2_fp32_vs_fp64"] = _rel_l2(states["fp64"], states["fp32"], _areas(g))
+    area = _areas(g)
+    out["rel_l2_fp32_vs_fp64"] = _rel_l2(states["fp64"], states["fp32"], area)
+    out["rel_l2_bf16_vs_fp64"] = _rel_l2(states["fp64"], states["bf16"], area)
     return out
 
 
@@ -296,13 +305,13 @@ def main():
     ap.add_argument("--cpu-steps-1t", type=int, default=3, help="oracle steps of the one-thread CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip every extra leg")
-    ap.add_argument("--extra", default="p2_2M,batch",
-                    help="comma-separated extra legs: p2_2M, batch, batch_fp32, riemann, p3 "
+    ap.add_argument("--extra", default="p2_2M,batch,batch_fp32",
+                    help="comma-separated extra legs: p2_2M, batch, batch_fp32, batch_bf16, riemann, p3 "
                          "(riemann runs 2 x ~25k steps to t = 0.3 and p3 builds an 8.4M-triangle mesh)")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-MFMA leg")
-    ap.add_argument("--form", choices=("fp64", "fp32"), default="fp64",
-                    help="form of the main timed leg (fp32: the FP32-MFMA precision form; its "
-                         "comparison leg is then skipped)")
+    ap.add_argument("--form", choices=("fp64", "fp32", "bf16"), default="fp64",
+                    help="form of the main timed leg (fp32 / bf16: the precision form with FP32 or BF16 "
+                         "contractions; the comparison legs are then skipped)")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batched leg")
     ap.add_argument("--riemann-t-end", type=float, default=0.3, help="end time of the riemann leg")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the stream-copy measurement")
@@ -318,7 +327,7 @@ def main():
     Np = (P + 1) * (P + 2) // 2
     g = workload(args.n, P)
     E = g.n_elements
-    fp32 = args.form == "fp32"
+    lowp = args.form != "fp64"
     cfl = 0.3 if P <= 2 else 0.2  # P3: the large-mesh P3 workload's CFL number
     s = Solver(g, degree=P, flux="lf", dt_policy="max", cfl=cfl, precision=args.form)
     s.project_ic(IC_VORTEX, vortex_params())
@@ -358,7 +367,7 @@ def main():
     per_launch = B * E
     avg_launch_ms = total_ms / (3 * args.steps)  # the stage kernels back to back, gaps included
     achieved = per_launch / (avg_launch_ms / 1e3) / 1e9
-    kfam = ("k_stage_pz" if P == 3 else "k_stage_px") if fp32 else ("k_stage_rp" if P == 1 else "k_stage")
+    kfam = ("k_stage_pz" if P == 3 else "k_stage_px") if lowp else ("k_stage_rp" if P == 1 else "k_stage")
     kname = f"{kfam}_p{P}"
     traffic = read_traffic(f"{kname}_lf_{args.n}x{args.n}_hbm.md")
 
@@ -378,10 +387,12 @@ def main():
                "cpu_model": hc["cpu_model"], "nproc": hc["nproc"], "affinity_cpus": hc["affinity_cpus"],
                "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
 
-    prec = None
-    if not args.no_precision and not fp32:
-        prec = precision_point(g, P, args.steps)
-        progress("precision form leg done")
+    prec, prec_bf16 = None, None
+    if not args.no_precision and not lowp:
+        prec = precision_point(g, P, args.steps, "fp32")
+        progress("precision form (fp32) leg done")
+        prec_bf16 = precision_point(g, P, args.steps, "bf16")
+        progress("precision form (bf16) leg done")
 
     extra = {}
     if not args.no_extra:
@@ -389,6 +400,7 @@ def main():
         runners = {"p2_2M": ("p2_2M_fp64", lambda: large_p2_point(1024, 20)),
                    "batch": ("batch_fp64", lambda: batch_point(args.batch, 100)),
                    "batch_fp32": ("batch_fp32_mfma", lambda: batch_point(args.batch, 100, precision="fp32")),
+                   "batch_bf16": ("batch_bf16_mfma", lambda: batch_point(args.batch, 100, precision="bf16")),
                    "riemann": ("riemann_p1_2M", lambda: riemann_point(1024, args.riemann_t_end)),
                    "p3": ("vortex_p3_8M", lambda: p3_point(2048, 50))}
         for leg in legs:
@@ -410,12 +422,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if fp32 else "f64",
+        "dtype": {"fp64": "f64", "fp32": "f32+f64", "bf16": "bf16+f32+f64"}[args.form],
         "data": "synthetic (seeded jittered split-quad mesh, projected isentropic vortex)",
         "config": {"workload": f"isentropic vortex, periodic [0,10]^2, {args.n}x{args.n} jittered split quads "
-                               f"({E} triangles), P{P}, LF, SSP-RK3 CFL {cfl}, " + ("FP32-MFMA" if fp32 else "FP64"),
+                               f"({E} triangles), P{P}, LF, SSP-RK3 CFL {cfl}, " +
+                               {"fp64": "FP64", "fp32": "FP32-MFMA", "bf16": "BF16-MFMA"}[args.form],
                    "elements": E, "degree": P, "dofs": dofs,
-                   "form": "precision (FP32-MFMA contractions, FP64 state)" if fp32 else "plain FP64"},
+                   "form": {"fp64": "plain FP64", "fp32": "precision (FP32-MFMA contractions, FP64 state)",
+                            "bf16": "precision (BF16-MFMA contractions, FP64 state)"}[args.form]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"{kfam}<{P},LF>", "bytes_per_launch": per_launch,
@@ -424,6 +438,7 @@ def main():
                      "measured_copy_GBs": copy_gbs},
         "cpu_baseline": cpu,
         "precision_form": prec,
+        "precision_form_bf16": prec_bf16,
         "wall_s": wall,
         "l2_vs_exact": err["l2"],
         "linf_vs_exact": err["linf"],

@@ -153,8 +153,10 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     const rkdg_params& p0 = params[0];
     if (p0.degree < 1 || p0.degree > 3) return bfail(nullptr, RKDG_ERR_ARG, "degree");
     if (p0.limiter != RKDG_LIMITER_NONE) return bfail(nullptr, RKDG_ERR_ARG, "the batching form runs without limiter");
-    if (p0.precision != RKDG_PREC_FP64 && p0.precision != RKDG_PREC_FP32_MFMA)
-        return bfail(nullptr, RKDG_ERR_ARG, "precision: RKDG_PREC_FP64 or RKDG_PREC_FP32_MFMA");
+    if (p0.precision != RKDG_PREC_FP64 && p0.precision != RKDG_PREC_FP32_MFMA && p0.precision != RKDG_PREC_BF16_MFMA)
+        return bfail(nullptr, RKDG_ERR_ARG, "precision: RKDG_PREC_FP64, RKDG_PREC_FP32_MFMA or RKDG_PREC_BF16_MFMA");
+    if (p0.precision == RKDG_PREC_BF16_MFMA && p0.degree > 2)
+        return bfail(nullptr, RKDG_ERR_ARG, "RKDG_PREC_BF16_MFMA in a batch: degree 1 or 2 (the staged tiles)");
     if (p0.block != 0 && (p0.block < 64 || p0.block > 256 || p0.block % 64 != 0 || 256 % p0.block != 0))
         return bfail(nullptr, RKDG_ERR_ARG, "block: 0 (library default) or 64, 128, 256");
     // The instances share one launch, so everything but the mesh, the initial state and
@@ -205,7 +207,7 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     b->P = p0.degree;
     b->Np = np_of(b->P);
     b->block = p0.block > 0 ? p0.block : 256;
-    if (p0.precision == RKDG_PREC_FP32_MFMA) b->block = 64;  // 64 elements on 256 threads
+    if (p0.precision != RKDG_PREC_FP64) b->block = 64;  // 64 elements on 256 threads
     b->xcd_remap = 1;
     b->E.resize(n);
     b->off.resize(n);
@@ -363,7 +365,11 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     }
     // precision form: staged tiles, one lane per element, when every halo fits (else four
     // lanes per element on 64-element tiles)
-    if (p0.precision == RKDG_PREC_FP32_MFMA && b->staged_ok) {
+    if (p0.precision == RKDG_PREC_BF16_MFMA && !b->staged_ok) {
+        rkdg_batch_destroy(b);
+        return bfail(nullptr, RKDG_ERR_ARG, "RKDG_PREC_BF16_MFMA: a tile's halo does not fit the staged LDS image");
+    }
+    if (p0.precision != RKDG_PREC_FP64 && b->staged_ok) {
         b->block = staged_block(b->P);
         b->staged = 1;
     }
@@ -556,6 +562,7 @@ static int device_cus(int dev) {
 static bool tiles_valid(const rkdg_batch* b, int staged, int block, int per_cu) {
     if (per_cu != 0 && per_cu != 4 && per_cu != 8) return false;
     if (staged) return b->staged_ok && b->P <= 2 && block == staged_block(b->P);
+    if (b->prm.precision == RKDG_PREC_BF16_MFMA) return false;  // staged tiles only
     if (b->prm.precision == RKDG_PREC_FP32_MFMA) return block == 64;
     return block == 64 || block == 128 || block == 256;
 }

@@ -185,18 +185,23 @@ static StageArgs make_args(rkdg_ctx* c, int stage, int slot, int dt_mode, double
     return a;
 }
 
-static int launch_one_stage(rkdg_ctx* c, int s, int slot, int dt_mode, double dt, double t_end) {
-    StageArgs a = make_args(c, s, slot, dt_mode, dt, t_end);
-    int rc = launch_stage(c->P, c->prm.flux, c->prm.precision, c->block, a, c->stream);
+// The stage kernel of a and, with the TVB limiter, the limiter pass over its output.
+static int launch_stage_and_limiter(rkdg_ctx* c, const StageArgs& a, hipStream_t stream) {
+    int rc = launch_stage(c->P, c->prm.flux, c->prm.precision, c->block, a, stream);
     if (rc) return fail(c, RKDG_ERR_HIP, std::string("stage kernel: ") + hip_error_string(rc));
     if (c->prm.limiter == RKDG_LIMITER_TVB) {
         LimArgs la{};
         la.U = a.Uout; la.lim = c->lim; la.pairs = c->pairs; la.nbr = c->nbr; la.flags = c->flags;
         la.ld = c->ld; la.E = c->E; la.M = c->prm.tvb_M; la.gamma = c->prm.gamma;
-        rc = launch_limiter(c->P, la, c->block, c->stream);
+        la.st = a.st; la.slot = a.slot; la.t_end = a.t_end;
+        rc = launch_limiter(c->P, la, c->block, stream);
         if (rc) return fail(c, RKDG_ERR_HIP, std::string("limiter kernel: ") + hip_error_string(rc));
     }
     return RKDG_OK;
+}
+
+static int launch_one_stage(rkdg_ctx* c, int s, int slot, int dt_mode, double dt, double t_end) {
+    return launch_stage_and_limiter(c, make_args(c, s, slot, dt_mode, dt, t_end), c->stream);
 }
 
 static int launch_step(rkdg_ctx* c, int slot, double t_end) {
@@ -274,8 +279,8 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
         return fail(nullptr, RKDG_ERR_ARG, "mesh arrays missing");
     if (prm->degree < 1 || prm->degree > 3) return fail(nullptr, RKDG_ERR_ARG, "degree must be 1, 2 or 3");
     if (prm->flux != RKDG_FLUX_LF && prm->flux != RKDG_FLUX_ROE) return fail(nullptr, RKDG_ERR_ARG, "flux");
-    if (prm->precision != RKDG_PREC_FP64 && prm->precision != RKDG_PREC_FP32_MFMA)
-        return fail(nullptr, RKDG_ERR_ARG, "precision: RKDG_PREC_FP64 or RKDG_PREC_FP32_MFMA");
+    if (prm->precision != RKDG_PREC_FP64 && prm->precision != RKDG_PREC_FP32_MFMA && prm->precision != RKDG_PREC_BF16_MFMA)
+        return fail(nullptr, RKDG_ERR_ARG, "precision: RKDG_PREC_FP64, RKDG_PREC_FP32_MFMA or RKDG_PREC_BF16_MFMA");
     if (!(prm->cfl > 0) && prm->dt_policy != RKDG_DT_FIXED) return fail(nullptr, RKDG_ERR_ARG, "cfl");
     if (prm->dt_policy == RKDG_DT_FIXED && !(prm->dt_fixed > 0)) return fail(nullptr, RKDG_ERR_ARG, "dt_fixed");
     if (m->n_elements > (1 << 29) - 1) return fail(nullptr, RKDG_ERR_ARG, "too many elements");
@@ -287,8 +292,9 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     c->block = prm->block > 0 ? prm->block : env_int("RKDG_BLOCK", prm->degree >= 3 ? 64 : 256);
     // precision form: staged tiles of 256 elements (lane per element), else 64 elements on
     // 256 threads (four lanes per element)
-    const bool px = prm->precision == RKDG_PREC_FP32_MFMA && env_int("RKDG_PX", 1);
-    if (prm->precision == RKDG_PREC_FP32_MFMA) c->block = px ? px_block(c->P) : 64;
+    const bool bf16 = prm->precision == RKDG_PREC_BF16_MFMA;
+    const bool px = bf16 || (prm->precision == RKDG_PREC_FP32_MFMA && env_int("RKDG_PX", 1));
+    if (prm->precision != RKDG_PREC_FP64) c->block = px ? px_block(c->P) : 64;
     if (c->block % 64 || c->block < 64 || c->block > 256) { delete c; return fail(nullptr, RKDG_ERR_ARG, "block must be 64..256, multiple of 64"); }
     c->xcd_remap = env_int("RKDG_XCD_REMAP", 1);
     c->ntiles = (c->E + c->block - 1) / c->block;
@@ -445,8 +451,12 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
             c->stagger = env_int("RKDG_STAGGER", 0);
         }
     }
-    if (px && !c->staged) {  // a halo did not fit: four lanes per element
-        c->block = 64;
+    if (px && !c->staged) {
+        if (bf16) {  // the BF16 contractions run on the staged tiles only
+            delete c;
+            return fail(nullptr, RKDG_ERR_ARG, "RKDG_PREC_BF16_MFMA: a tile's halo does not fit the staged LDS image");
+        }
+        c->block = 64;  // a halo did not fit: four lanes per element
         c->ntiles = (c->E + c->block - 1) / c->block;
     }
 
@@ -848,6 +858,7 @@ int rkdg_time_advance(rkdg_ctx* c, int32_t nsteps, double* out) {
                 LimArgs la{};
                 la.U = a.Uout; la.lim = c->lim; la.pairs = c->pairs; la.nbr = c->nbr; la.flags = c->flags;
                 la.ld = c->ld; la.E = c->E; la.M = c->prm.tvb_M; la.gamma = c->prm.gamma;
+                la.st = a.st; la.slot = a.slot; la.t_end = 0.0;
                 HIPCHK(c, hipEventRecord(ev[2 + 2 * n + 2 * k], c->stream));
                 rc = launch_limiter(c->P, la, c->block, c->stream);
                 if (rc) return fail(c, RKDG_ERR_HIP, hip_error_string(rc));

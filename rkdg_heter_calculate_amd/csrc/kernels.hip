@@ -1063,6 +1063,7 @@ __global__ void __launch_bounds__(256) k_limit(LimArgs a) {
     const DevTables& T = c_tab[P - 1];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.E) return;
+    if (a.t_end > 0 && !(a.t_end - a.st->t[a.slot] > 0)) return;  // past the end time (as its stage)
     double c[4][Np], nm[3][4];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
@@ -1245,45 +1246,59 @@ static void launch_stage_mx_p(int flux, const StageArgs& a, hipStream_t s) {
     else hipLaunchKernelGGL((k_stage_mx<P, RKDG_FLUX_LF>), grid, blk, 0, s, a);
 }
 
-template <int P, int FLUX>
+template <int P, int FLUX, int CT>
 static void launch_stage_px_pf(const StageArgs& a, hipStream_t s) {
     constexpr size_t lds = (size_t)4 * Deg<P>::Np * staged_stride(P) * sizeof(double);
     static bool opted = false;  // above 64 KiB the kernel must opt in
     if (lds > 65536 && !opted) {
-        hipFuncSetAttribute((const void*)k_stage_px<P, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipFuncSetAttribute((const void*)k_stage_px<P, FLUX, CT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         opted = true;
     }
-    hipLaunchKernelGGL((k_stage_px<P, FLUX>), dim3(grid_blocks(a)), dim3(staged_block(P)), lds, s, a);
+    hipLaunchKernelGGL((k_stage_px<P, FLUX, CT>), dim3(grid_blocks(a)), dim3(staged_block(P)), lds, s, a);
 }
 
-template <int P>
+template <int P, int CT>
 static void launch_stage_px_p(int flux, const StageArgs& a, hipStream_t s) {
-    if (flux == RKDG_FLUX_ROE) launch_stage_px_pf<P, RKDG_FLUX_ROE>(a, s);
-    else launch_stage_px_pf<P, RKDG_FLUX_LF>(a, s);
+    if (flux == RKDG_FLUX_ROE) launch_stage_px_pf<P, RKDG_FLUX_ROE, CT>(a, s);
+    else launch_stage_px_pf<P, RKDG_FLUX_LF, CT>(a, s);
 }
 
-template <int FLUX>
+template <int FLUX, int CT>
 static void launch_stage_pz_f(const StageArgs& a, hipStream_t s) {
     static bool opted = false;  // above 64 KiB the kernel must opt in
     if (!opted) {
-        hipFuncSetAttribute((const void*)k_stage_pz<3, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Pz<3>::LDS);
+        hipFuncSetAttribute((const void*)k_stage_pz<3, FLUX, CT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Pz<3>::LDS);
         opted = true;
     }
-    hipLaunchKernelGGL((k_stage_pz<3, FLUX>), dim3(grid_blocks(a)), dim3(Pz<3>::B), Pz<3>::LDS, s, a);
+    hipLaunchKernelGGL((k_stage_pz<3, FLUX, CT>), dim3(grid_blocks(a)), dim3(Pz<3>::B), Pz<3>::LDS, s, a);
 }
 
 int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream) {
     hipStream_t s = (hipStream_t)stream;
+    if (prec == RKDG_PREC_BF16_MFMA) {
+        // BF16 contractions: staged tiles of 256 elements, one lane per element (P1, P2)
+        if (!a.staged || block != px_block(P)) return (int)hipErrorInvalidValue;
+        switch (P) {
+            case 1: launch_stage_px_p<1, CT_BF16>(flux, a, s); break;
+            case 2: launch_stage_px_p<2, CT_BF16>(flux, a, s); break;
+            case 3:
+                if (flux == RKDG_FLUX_ROE) launch_stage_pz_f<RKDG_FLUX_ROE, CT_BF16>(a, s);
+                else launch_stage_pz_f<RKDG_FLUX_LF, CT_BF16>(a, s);
+                break;
+            default: return (int)hipErrorInvalidValue;
+        }
+        return (int)hipGetLastError();
+    }
     if (prec == RKDG_PREC_FP32_MFMA) {
         // staged tiles of 256 elements, one lane per element (P1, P2)
         if (a.staged) {
             if (block != px_block(P)) return (int)hipErrorInvalidValue;
             switch (P) {
-                case 1: launch_stage_px_p<1>(flux, a, s); break;
-                case 2: launch_stage_px_p<2>(flux, a, s); break;
+                case 1: launch_stage_px_p<1, CT_F32>(flux, a, s); break;
+                case 2: launch_stage_px_p<2, CT_F32>(flux, a, s); break;
                 case 3:
-                    if (flux == RKDG_FLUX_ROE) launch_stage_pz_f<RKDG_FLUX_ROE>(a, s);
-                    else launch_stage_pz_f<RKDG_FLUX_LF>(a, s);
+                    if (flux == RKDG_FLUX_ROE) launch_stage_pz_f<RKDG_FLUX_ROE, CT_F32>(a, s);
+                    else launch_stage_pz_f<RKDG_FLUX_LF, CT_F32>(a, s);
                     break;
                 default: return (int)hipErrorInvalidValue;
             }

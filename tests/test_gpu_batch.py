@@ -5,7 +5,8 @@ the dt of src/kernelfunctions.cu:3-81; a batch must reproduce, for every instanc
 the oracle computes for that mesh alone: uneven sizes (1 to 13 layout tiles), per-instance
 vortex strength, Mach number, flow angle and CFL number, after the autotuner and under
 every tile configuration it can pick.  Bars: 1e-12 relative L2 per variable (FP64),
-1e-6 (FP32-MFMA contractions).
+1e-6 (FP32-MFMA contractions), 1e-3 (BF16-MFMA contractions: 8 significant bits on the
+deviations, and the strongest vortices of the set on 7x7 to 20x20 meshes).
 """
 import math
 
@@ -19,7 +20,7 @@ from helpers import rel_l2
 
 pytestmark = pytest.mark.gpu
 STEPS = 6
-TOL = {"fp64": 1e-12, "fp32": 1e-6}
+TOL = {"fp64": 1e-12, "fp32": 1e-6, "bf16": 1e-3}
 CFL = {1: 0.25, 2: 0.22, 3: 0.14}
 
 
@@ -66,7 +67,9 @@ def check_against_oracle(b, inst, ref, prec, label):
         err = rel_l2(b.get_state(i), Uo, M)
         worst = np.maximum(worst, err)
         assert np.all(err < TOL[prec]), (label, i, inst[i][0].n_elements, err)
-        assert abs(b.time(i) - to) <= 1e-13 * to, (label, i, b.time(i), to)
+        # the time sums the CFL dts of the instance's own cell means: 1e-13 in FP64; in the
+        # precision form the means carry the contractions' rounding
+        assert abs(b.time(i) - to) <= (1e-13 if prec == "fp64" else TOL[prec]) * to, (label, i, b.time(i), to)
     print(label, "worst relative L2 per variable", worst, flush=True)
 
 
@@ -77,7 +80,8 @@ def test_instance_set_spans_tile_counts():
     assert len({round(vp[1], 6) for _, vp, _ in inst}) == len(inst)  # per-instance beta
 
 
-@pytest.mark.parametrize("P,prec", [(2, "fp64"), (2, "fp32"), (1, "fp64"), (3, "fp64"), (1, "fp32"), (3, "fp32")])
+@pytest.mark.parametrize("P,prec", [(2, "fp64"), (2, "fp32"), (1, "fp64"), (3, "fp64"), (1, "fp32"), (3, "fp32"),
+                                    (2, "bf16"), (1, "bf16")])
 def test_autotuned_batch_matches_oracle(P, prec):
     print(f"case: autotuned batch P{P} {prec}", flush=True)
     inst, ref = oracle_states(P)
@@ -93,7 +97,8 @@ def test_autotuned_batch_matches_oracle(P, prec):
 
 CONFIGS = [("fp64", 1, 256, pw) for pw in (0, 4, 8)] + \
           [("fp64", 0, blk, pw) for blk in (64, 128, 256) for pw in (0, 4, 8)] + \
-          [("fp32", 1, 256, pw) for pw in (0, 4, 8)] + [("fp32", 0, 64, pw) for pw in (0, 4, 8)]
+          [("fp32", 1, 256, pw) for pw in (0, 4, 8)] + [("fp32", 0, 64, pw) for pw in (0, 4, 8)] + \
+          [("bf16", 1, 256, pw) for pw in (0, 4, 8)]
 
 
 @pytest.mark.parametrize("prec,staged,block,per_cu", CONFIGS)
