@@ -139,6 +139,21 @@ int rkdg_advance(rkdg_ctx* ctx, int32_t nsteps, double t_end, int32_t* steps_don
  * src/rkdgadvance.cpp:201-213). */
 int rkdg_residual(rkdg_ctx* ctx, double res[2]);
 
+/* One SSP-RK3 step of length dt > 0 on the context's state that allocates nothing: no
+ * reference counterpart, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It runs the kernels
+ * of rkdg_stage(0, 1, 2, dt) (and the limiter) on `stream` (NULL: the context's stream)
+ * without synchronising; their scratch — the step counters and one pair of residual maxima
+ * per workgroup tile, which rkdg_create otherwise allocates per context — is a static
+ * device array of the library (one per GPU, kept until the process ends) of
+ * RKDG_NOALLOC_MAX_TILES tiles (16.7M elements at 256 per tile, 8.4M at 128): a larger mesh
+ * is rejected with RKDG_ERR_ARG.  res_dev (device memory, may be NULL) receives the step's
+ * residual pair, as rkdg_residual reports it.  The context's simulated time and step count
+ * do not advance and non-physical states are not reported (nothing is read back).  Calls
+ * share their GPU's scratch: calls that overlap on one GPU race. */
+#define RKDG_NOALLOC_MAX_TILES 65536
+int rkdg_step_noalloc(rkdg_ctx* ctx, double dt, double* res_dev, void* stream);
+
 /* Simulated time and steps taken so far (synchronises). */
 int rkdg_time(rkdg_ctx* ctx, double* t, int64_t* steps);
 

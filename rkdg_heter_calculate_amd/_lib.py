@@ -78,6 +78,7 @@ SIGNATURES = [
     ("rkdg_info", c_int32, [ctypes.c_void_p, P_int64]),
     ("rkdg_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
     ("rkdg_strerror", ctypes.c_char_p, [c_int32]),
+    ("rkdg_step_noalloc", c_int32, [ctypes.c_void_p, c_double, ctypes.c_void_p, ctypes.c_void_p]),
     ("rkdg_batch_create", c_int32, [c_int32, ctypes.POINTER(Mesh), ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
     ("rkdg_batch_destroy", c_int32, [ctypes.c_void_p]),
     ("rkdg_batch_project_ic", c_int32, [ctypes.c_void_p, c_int32, c_int32, P_double, c_int32]),

@@ -114,6 +114,8 @@ struct rkdg_ctx {
     int* flags = nullptr;
     DevState* st = nullptr;
     double* res_part = nullptr;  // residual maxima per tile [ntiles][2], written by stage 2
+    void* scratch = nullptr;     // this device's static scratch of rkdg_step_noalloc (not owned)
+    int scratch_rc = 0;          // hipError_t of resolving it
     int cur = 0;            // buffer holding the latest state
     int next_stage = 0;     // rkdg_stage protocol
     long long steps = 0;    // steps launched (host count; parity of the step slot)
@@ -464,6 +466,8 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     c->dev = prm->device;
     CREATECHK(hipSetDevice(c->dev));
     CREATECHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // the allocation-free step's scratch on this device (resolved once per device)
+    c->scratch_rc = noalloc_scratch(c->dev, &c->scratch);
     size_t ubytes = sizeof(double) * (size_t)4 * c->Np * ld;
     // the kernels address every array with 32-bit byte offsets
     if (ubytes >= (1ull << 32) || sizeof(double) * (size_t)GEO_ROWS * ld >= (1ull << 32))
@@ -641,6 +645,38 @@ int rkdg_stage(rkdg_ctx* c, int32_t s, double dt) {
     c->cur = s == 2 ? 0 : s + 1;
     c->next_stage = (s + 1) % 3;
     if (s == 2) c->steps++;
+    return RKDG_OK;
+}
+
+int rkdg_step_noalloc(rkdg_ctx* c, double dt, double* res_dev, void* stream) {
+    if (!c || !(dt > 0)) return fail(c, RKDG_ERR_ARG, "step_noalloc: dt must be positive");
+    if (!c->have_state) return fail(c, RKDG_ERR_STATE, "no state");
+    if (c->next_stage != 0) return fail(c, RKDG_ERR_STATE, "a step started with rkdg_stage is unfinished");
+    if (c->ntiles > NOALLOC_MAX_TILES)
+        return fail(c, RKDG_ERR_ARG, "step_noalloc: " + std::to_string(c->ntiles) + " workgroup tiles exceed the scratch's " +
+                                         std::to_string(NOALLOC_MAX_TILES));
+    if (!c->scratch)
+        return fail(c, RKDG_ERR_HIP, std::string("step_noalloc scratch: ") + hip_error_string(c->scratch_rc));
+    HIPCHK(c, hipSetDevice(c->dev));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    DevState* st = (DevState*)c->scratch;
+    double* part = (double*)((char*)c->scratch + 256);
+    // the counters start cleared (a fill kernel: nothing here may record a memset into a
+    // caller's graph); every tile writes its residual pair in stage 2
+    int rc = launch_state_init(st, 0, 0.0, 1, s);
+    if (rc) return fail(c, RKDG_ERR_HIP, hip_error_string(rc));
+    for (int stage = 0; stage < 3; ++stage) {
+        StageArgs a = make_args(c, stage, 0, DTM_HOST, dt, 0.0);
+        a.st = st;
+        a.res_part = part;
+        rc = launch_stage_and_limiter(c, a, s);
+        if (rc) return rc;
+    }
+    if (res_dev) {
+        rc = launch_res_reduce(part, c->ntiles, res_dev, s);
+        if (rc) return fail(c, RKDG_ERR_HIP, hip_error_string(rc));
+    }
+    c->cur = 0;
     return RKDG_OK;
 }
 

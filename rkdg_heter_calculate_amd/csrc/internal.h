@@ -230,6 +230,15 @@ int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void*
 int launch_wavespeed(int P, const StageArgs& a, int block, void* stream);
 int launch_state_init(DevState* st, int slot, double t, int n, void* stream);
 int launch_ws_clear(DevState* st, int slot, int n, void* stream);
+// Allocation-free step (rkdg_step_noalloc): its scratch, one static device array per GPU
+// holding a DevState in the first 256 bytes and then NOALLOC_MAX_TILES residual pairs.
+constexpr int NOALLOC_MAX_TILES = RKDG_NOALLOC_MAX_TILES;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+constexpr size_t NOALLOC_SCRATCH_BYTES = 256 + sizeof(double) * 2 * (size_t)NOALLOC_MAX_TILES;
+// The scratch's device address on device dev (dev must be current); resolved once per
+// device with hipGetSymbolAddress.  Returns a hipError_t.
+int noalloc_scratch(int dev, void** out);
+int launch_res_reduce(const double* part, int ntiles, double* out, void* stream);
 int launch_limiter(int P, const LimArgs& a, int block, void* stream);
 int launch_stream_copy(const void* src, void* dst, long long bytes, void* stream);
 const char* hip_error_string(int code);

@@ -21,6 +21,7 @@
 // XCD sweeps one contiguous range of elements and neighbour gathers hit its own L2.
 #include <hip/hip_runtime.h>
 #include <climits>
+#include <mutex>
 #include <cstdlib>
 #include <type_traits>
 #include "internal.h"
@@ -1332,6 +1333,63 @@ int launch_wavespeed(int P, const StageArgs& a, int block, void* stream) {
         case 3: hipLaunchKernelGGL(k_wavespeed<3>, grid, blk, 0, s, a); break;
         default: return (int)hipErrorInvalidValue;
     }
+    return (int)hipGetLastError();
+}
+
+// ---- allocation-free step: its scratch lives in one static device array -------------
+// Layout: a DevState (the step's counters and wave-speed maxima) in the first 256 bytes,
+// then one pair of residual maxima per workgroup tile.  Each GPU holds its own copy of the
+// array until the process ends; host code never names it in a kernel argument or copy
+// (on the host the name is a stand-in, not a device address): it uses the address
+// hipGetSymbolAddress returned for the device.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+int noalloc_scratch(int dev, void** out) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        // with dev current: this device's copy of the array
+        void* ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess) return (int)rc;
+        g_scratch_dev[dev] = ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return (int)hipSuccess;
+}
+
+// Maxima of the per-tile residual pairs into out[0..1] (one workgroup).
+__global__ void __launch_bounds__(256) k_res_reduce(const double* __restrict__ part, int ntiles,
+                                                    double* __restrict__ out) {
+    __shared__ double red[2][256];
+    double a = 0.0, b = 0.0;
+    for (int t = threadIdx.x; t < ntiles; t += blockDim.x) {
+        a = fmax(a, part[2 * t]);
+        b = fmax(b, part[2 * t + 1]);
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+    }
+}
+
+int launch_res_reduce(const double* part, int ntiles, double* out, void* stream) {
+    hipLaunchKernelGGL(k_res_reduce, dim3(1), dim3(256), 0, (hipStream_t)stream, part, ntiles, out);
     return (int)hipGetLastError();
 }
 
