@@ -683,8 +683,9 @@ struct Staged {
 
 // This lane's halo ids of a tile (element hb * 32 + lane / 2 of the halo, chunk hb).
 template <int P>
-__device__ __forceinline__ int staged_halo_ids(const StageArgs& a, int tile, int (&hid)[Staged<P>::HCH]) {
-    const int lane = threadIdx.x & 63;
+__device__ __forceinline__ int staged_halo_ids(const StageArgs& a, int tile, int (&hid)[Staged<P>::HCH],
+                                               int tid = threadIdx.x) {
+    const int lane = tid & 63;
     const int h0 = a.hoff[tile], nh = a.hoff[tile + 1] - h0;
 #pragma unroll
     for (int hb = 0; hb < Staged<P>::HCH; ++hb) {
@@ -697,9 +698,9 @@ __device__ __forceinline__ int staged_halo_ids(const StageArgs& a, int tile, int
 // Issues this wave's share of the direct loads of a tile's image.
 template <int P>
 __device__ __forceinline__ void staged_fill(const StageArgs& a, int tile, int nh, const int (&hid)[Staged<P>::HCH],
-                                            double* sU) {
+                                            double* sU, int tid = threadIdx.x) {
     using L = Staged<P>;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = tid & 63, wave = tid >> 6;
     char* lds = (char*)sU;
     // the tile's rows inside its layout tile: row r starts 2 KiB after row r - 1
     const long long e0 = (long long)tile * L::B;
@@ -728,28 +729,29 @@ __device__ __forceinline__ void staged_fill(const StageArgs& a, int tile, int nh
 // One workgroup tile, operands loaded from global memory (STAGED: neighbour traces from
 // the tile's LDS image).
 template <int P, int FLUX, bool STAGED>
-__device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& ti, const double* sVe, double* sU) {
+__device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& ti, const double* sVe, double* sU,
+                                           int tid) {
     constexpr int Np = Deg<P>::Np;
     bool done;
     const double dt = stage_dt(a, ti, done);
     stage_bookkeeping(a, ti, dt, done);
     if (done) return;
 
-    const int e = ti.tile * blockDim.x + threadIdx.x;
+    const int e = ti.tile * blockDim.x + tid;
     double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
     double Uo[4][Np];
     ElemOps o{};
     if (STAGED) {
         int hid[Staged<P>::HCH];
-        const int nh = staged_halo_ids<P>(a, ti.tile, hid);
-        staged_fill<P>(a, ti.tile, nh, hid, sU);
+        const int nh = staged_halo_ids<P>(a, ti.tile, hid, tid);
+        staged_fill<P>(a, ti.tile, nh, hid, sU, tid);
         if (e < a.E) load_element_ops(a, a.nloc, e, o);  // same round trip as the image
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int i = 0; i < Np; ++i) Uo[v][i] = sU[(v * Np + i) * Staged<P>::S + threadIdx.x];
+            for (int i = 0; i < Np; ++i) Uo[v][i] = sU[(v * Np + i) * Staged<P>::S + tid];
     } else if (e < a.E) {
 #pragma unroll
         for (int v = 0; v < 4; ++v)
@@ -991,7 +993,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         TileInfo ti;
         if (!tile_of_iter(a, it, nit, ti)) continue;  // uniform across the workgroup
-        stage_tile<P, FLUX, STAGED>(a, ti, sVe, sU);
+        stage_tile<P, FLUX, STAGED>(a, ti, sVe, sU, (int)threadIdx.x);
         __syncthreads();  // LDS and the reduction scratch are reused by the next tile
     }
 }
