@@ -137,6 +137,11 @@ void build_ref_basis(int P, RefBasis& B) {
             for (int i = 0; i < Np; ++i) T.Ve[k][j][i] = psi_value(B, i, px, py);
         }
     }
+    // psi_0 = 1 exactly (G_00 = 1 in the mean inner product): the kernels take mode 0 as the
+    // constant term of a point value without multiplying it by the table
+    for (int q = 0; q < nq; ++q) T.Vv[q][0] = 1.0;
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < ne; ++j) T.Ve[k][j][0] = 1.0;
     // limiter: P1 modes at the edge midpoints and the least-squares slope recovery
     double A[3][2];
     for (int k = 0; k < 3; ++k) {

@@ -339,7 +339,7 @@ __device__ __forceinline__ void element_volume(const StageArgs& a, const double 
         double uq[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            double s = Uo[v][0] * T.Vv[q][0];
+            double s = Uo[v][0];  // psi_0 = 1 (the mean)
 #pragma unroll
             for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Vv[q][i], s);
             uq[v] = s;
@@ -449,7 +449,7 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
                     for (int j = 0; j < Nqe; ++j) {
                         const double t = tb[(Nqe - 1 - j) * Np + i];
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) uRa[j][v] = i == 0 ? c[v] * t : fma(c[v], t, uRa[j][v]);
+                        for (int v = 0; v < 4; ++v) uRa[j][v] = i == 0 ? c[v] : fma(c[v], t, uRa[j][v]);
                     }
                 }
             }
@@ -458,7 +458,7 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
                 double uL[4], uR[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    double s = Uo[v][0] * T.Ve[k][j][0];
+                    double s = Uo[v][0];  // psi_0 = 1
 #pragma unroll
                     for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Ve[k][j][i], s);
                     uL[v] = s;
@@ -500,7 +500,7 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
                 double uL[4], uR[4];
     #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    double s = Uo[v][0] * T.Ve[k][j][0];
+                    double s = Uo[v][0];  // psi_0 = 1
     #pragma unroll
                     for (int i = 1; i < Np; ++i) s = fma(Uo[v][i], T.Ve[k][j][i], s);
                     uL[v] = s;
@@ -510,7 +510,7 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
                     const double* t = tb + (Nqe - 1 - j) * Np;
     #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        double s = (NBSRC != NB_GLOBAL ? src[(v * Np) * S] : Nb[v][0]) * t[0];
+                        double s = NBSRC != NB_GLOBAL ? src[(v * Np) * S] : Nb[v][0];  // psi_0 = 1
     #pragma unroll
                         for (int i = 1; i < Np; ++i) s = fma(NBSRC != NB_GLOBAL ? src[(v * Np + i) * S] : Nb[v][i], t[i], s);
                         uR[v] = s;
