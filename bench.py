@@ -368,7 +368,7 @@ def main():
     avg_launch_ms = total_ms / (3 * args.steps)  # the stage kernels back to back, gaps included
     achieved = per_launch / (avg_launch_ms / 1e3) / 1e9
     kfam = ("k_stage_pz" if P == 3 else "k_stage_px") if lowp else ("k_stage_rp" if P == 1 else "k_stage")
-    kname = f"{kfam}_p{P}"
+    kname = f"{kfam}_bf16_p{P}" if args.form == "bf16" else f"{kfam}_p{P}"
     traffic = read_traffic(f"{kname}_lf_{args.n}x{args.n}_hbm.md")
 
     cpu = None
