@@ -559,7 +559,7 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
 
 // SSP-RK3 combination, mass division and store of one element (stage 2 also folds its
 // CFL wave speed and residual into ws_l, r0_l, r1_l, running maxima).
-template <int P, int NBSRC>
+template <int P, int NBSRC, bool U0_IN_NB = false>
 __device__ __forceinline__ void element_update(const StageArgs& a, int e, const double (&Uo)[4][Deg<P>::Np],
                                                const ElemGeo& eg, ElemWork<P>& W, double dt, bool dtf,
                                                DevState* st, double& ws_l, double& r0_l, double& r1_l) {
@@ -577,7 +577,7 @@ __device__ __forceinline__ void element_update(const StageArgs& a, int e, const 
     // (buffer-resource addressing: the 4 Np rows cost no per-row address registers)
     const __amdgpu_buffer_rsrc_t rO = tm_rsrc(a.Uout, a.ld, 4 * Np);
     const unsigned vo = tm_voff(e, 4 * Np);
-    if (!PF && a.stage > 0) {
+    if (!PF && !U0_IN_NB && a.stage > 0) {
         const __amdgpu_buffer_rsrc_t r0 = tm_rsrc(a.U0, a.ld, 4 * Np);
 #pragma unroll
         for (int v = 0; v < 4; ++v)
@@ -768,15 +768,142 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
 }
 
 // Persistent staged form with the next tile's image in flight during the current tile's
-// SSP-RK3 update: once every wave has finished its residual (a barrier), the LDS image is
-// free, the direct loads of the next tile's image and its operands are issued, and only
-// then are the step-start modes read and the results stored.  The image's memory traffic
-// thus overlaps the update's instead of waiting for it.
-template <int P, int FLUX>
-__global__ void __launch_bounds__(256) k_stage_pipe(StageArgs a) {
-    using D = Deg<P>;
+// SSP-RK3 update.  Loads complete in order, so the order of issue decides what a wait waits
+// for: once every wave has finished its residual (a barrier), the update's step-start modes
+// are requested first, then the next tile's image and operands; the update then waits only
+// for its own loads — the image loads behind them have a fixed count per wave (every own-row
+// part and every halo chunk is issued, lanes past the halo's end reading element 0 into
+// unused slots; the operands are read at an index clamped into the mesh), so the compiler's
+// wait counts them instead of draining the queue.  The next tile's halo ids are requested
+// before the residual, so their values have arrived when the image addresses are formed.
+// The last tile of a workgroup runs without the prefetch (a separate instantiation of the
+// body, so that no path through the prefetching body lacks its loads).
+// Byte offset of halo element h's first row in the tile-major state, plus the lane's half of
+// the double (two lanes per double).
+template <int P>
+__device__ __forceinline__ unsigned halo_offset(int h) {
     using L = Staged<P>;
-    constexpr int Np = D::Np, Nqe = D::Nqe;
+    return (unsigned)((((h >> 8) * L::NR) * 256 + (h & 255)) * 8) + (unsigned)(threadIdx.x & 1) * 4u;
+}
+
+template <int P>
+__device__ __forceinline__ void staged_fill_fixed(const StageArgs& a, int tile, const unsigned (&hoff)[Staged<P>::HCH],
+                                                  double* sU) {
+    using L = Staged<P>;
+    static_assert((L::NR * L::PARTS) % L::NW == 0 && L::NR % L::NW == 0, "rows divide among the waves");
+    static_assert(L::S - L::B == 32 * L::HCH, "every halo chunk lands inside the halo slots");
+    // Addresses as a uniform base (scalar registers: the wave index, rows and LDS slots) plus
+    // one 32-bit lane offset, so that the loads cost almost no vector registers at the point
+    // of the update where they are issued.
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    char* lds = (char*)sU;
+    const char* U = (const char*)a.U;
+    const long long e0 = (long long)tile * L::B;
+    const char* own = U + (((e0 >> 8) * L::NR) * 256 + (e0 & 255)) * 8;
+    const unsigned lo = (unsigned)lane * 16u;
+#pragma unroll
+    for (int j = 0; j < L::NR * L::PARTS / L::NW; ++j) {
+        const int c = wave + j * L::NW;
+        const int r = c / L::PARTS, part = c - r * L::PARTS;
+        __builtin_amdgcn_global_load_lds((const void*)(own + r * 2048 + part * 1024 + lo),
+                                         (__attribute__((address_space(3))) void*)(lds + r * L::S * 8 + part * 1024), 16, 0, 0);
+    }
+#pragma unroll
+    for (int hb = 0; hb < L::HCH; ++hb) {
+        const unsigned ho = hoff[hb];
+#pragma unroll
+        for (int j = 0; j < L::NR / L::NW; ++j) {
+            const int r = wave + j * L::NW;
+            __builtin_amdgcn_global_load_lds((const void*)(U + (size_t)r * 2048 + ho),
+                                             (__attribute__((address_space(3))) void*)(lds + (r * L::S + L::B + hb * 32) * 8),
+                                             4, 0, 0);
+        }
+    }
+}
+
+// The next tile's operands as loaded (the dt flag raw: a test of it here would wait for
+// every load in flight), read at an index clamped into the mesh.
+struct ElemOpsRaw {
+    int w0, w1, w2;
+    ElemGeo eg;
+    int flag;
+};
+
+__device__ __forceinline__ void load_element_ops_clamped(const StageArgs& a, int e, ElemOpsRaw& o) {
+    e = min(e, a.E - 1);
+    o.w0 = tmx(a.nloc, 0, NBR_ROWS, e);
+    o.w1 = tmx(a.nloc, 1, NBR_ROWS, e);
+    o.w2 = tmx(a.nloc, 2, NBR_ROWS, e);
+    o.eg.area = tmx(a.geo, 4, GEO_ROWS, e);
+    o.eg.xix = tmx(a.geo, 0, GEO_ROWS, e);
+    o.eg.xiy = tmx(a.geo, 1, GEO_ROWS, e);
+    o.eg.etx = tmx(a.geo, 2, GEO_ROWS, e);
+    o.eg.ety = tmx(a.geo, 3, GEO_ROWS, e);
+    o.flag = a.dtflag[e];
+}
+
+// The next valid tile of this workgroup after iteration `it` (it advances).
+__device__ __forceinline__ bool next_tile(const StageArgs& a, int& it, int nit, TileInfo& t) {
+    for (it += gridDim.x; it < nit; it += gridDim.x)
+        if (tile_of_iter(a, it, nit, t)) return true;
+    return false;
+}
+
+template <int P, int FLUX, bool PF>
+__device__ __forceinline__ void pipe_tile(const StageArgs& a, const TileInfo& ti, const ElemOpsRaw& o, const double* sVe,
+                                          double* sU, const TileInfo& tn, ElemOpsRaw& on) {
+    using L = Staged<P>;
+    constexpr int Np = Deg<P>::Np;
+    const int tid = threadIdx.x;
+    bool done;
+    const double dt = stage_dt(a, ti, done);
+    stage_bookkeeping(a, ti, dt, done);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();  // the tile's image and operands have landed (and sVe on entry)
+    const int e = ti.tile * L::B + tid;
+    const bool live = e < a.E && !done;
+    const bool dtf = a.stage == 2 && o.flag != 0;
+    double Uo[4][Np];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int i = 0; i < Np; ++i) Uo[v][i] = sU[(v * Np + i) * L::S + tid];
+    int hid[L::HCH];
+    if constexpr (PF) staged_halo_ids<P>(a, tn.tile, hid);
+    ElemWork<P> W;
+    if (live) element_residual<P, FLUX, NB_LDS>(a, e, Uo, o.eg, o.w0, o.w1, o.w2, sVe, sU, W);
+    // the next image's halo offsets are formed here, before the step-start loads are issued,
+    // so that the wait for the halo ids cannot stop at those loads
+    unsigned hoff[L::HCH];
+    if constexpr (PF) {
+#pragma unroll
+        for (int hb = 0; hb < L::HCH; ++hb) {
+            hoff[hb] = halo_offset<P>(hid[hb]);
+            asm volatile("" ::"v"(hoff[hb]));  // formed here, not after the loads below
+        }
+    }
+    if constexpr (PF) load_element_ops_clamped(a, tn.tile * L::B + tid, on);
+    __syncthreads();  // every wave is done with the image
+    if (a.stage > 0) {
+        const __amdgpu_buffer_rsrc_t r0 = tm_rsrc(a.U0, a.ld, 4 * Np);
+        const unsigned vo = tm_voff(e, 4 * Np);
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int i = 0; i < Np; ++i) W.Nb[v][i] = tm_load(r0, vo, v * Np + i);
+    }
+    if constexpr (PF) staged_fill_fixed<P>(a, tn.tile, hoff, sU);
+    double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
+    if (live) element_update<P, NB_LDS, true>(a, e, Uo, o.eg, W, dt, dtf, ti.st, ws_l, r0_l, r1_l);
+    if (a.stage == 2)
+        block_stage_max(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], tile_residual(a, ti));
+}
+
+template <int P, int FLUX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_stage_pipe(StageArgs a) {
+    using L = Staged<P>;
+    constexpr int Np = Deg<P>::Np, Nqe = Deg<P>::Nqe;
     const DevTables& T = c_tab[P - 1];
     __shared__ double sVe[3 * Nqe * Np];
     extern __shared__ __align__(16) double sU[];
@@ -785,53 +912,28 @@ __global__ void __launch_bounds__(256) k_stage_pipe(StageArgs a) {
         sVe[t] = T.Ve[k][j][i];
     }
     const int nit = iter_count(a);
-    const int G = gridDim.x, tid = threadIdx.x;
-    int it = blockIdx.x;
+    int it = (int)blockIdx.x - (int)gridDim.x;
     TileInfo ti;
-    bool have = it < nit && tile_of_iter(a, it, nit, ti);
-    ElemOps o{};
-    if (have) {
+    if (!next_tile(a, it, nit, ti)) return;
+    ElemOpsRaw o;
+    {
         int hid[L::HCH];
-        const int nh = staged_halo_ids<P>(a, ti.tile, hid);
-        staged_fill<P>(a, ti.tile, nh, hid, sU);
-        const int e = ti.tile * L::B + tid;
-        if (e < a.E) load_element_ops(a, a.nloc, e, o);
+        staged_halo_ids<P>(a, ti.tile, hid);
+        unsigned hoff[L::HCH];
+#pragma unroll
+        for (int hb = 0; hb < L::HCH; ++hb) hoff[hb] = halo_offset<P>(hid[hb]);
+        staged_fill_fixed<P>(a, ti.tile, hoff, sU);
+        load_element_ops_clamped(a, ti.tile * L::B + (int)threadIdx.x, o);
     }
-    while (have) {
-        bool done;
-        const double dt = stage_dt(a, ti, done);
-        stage_bookkeeping(a, ti, dt, done);
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();  // the tile's image and operands have landed (and sVe on entry)
-        const int e = ti.tile * L::B + tid;
-        const bool live = e < a.E && !done;
-        double Uo[4][Np];
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int i = 0; i < Np; ++i) Uo[v][i] = sU[(v * Np + i) * L::S + tid];
-        it += G;
-        TileInfo tn;
-        const bool have_n = it < nit && tile_of_iter(a, it, nit, tn);
-        ElemWork<P> W;
-        if (live) element_residual<P, FLUX, NB_LDS_LEAN>(a, e, Uo, o.eg, o.w0, o.w1, o.w2, sVe, sU, W);
-        __syncthreads();  // every wave is done with the image
-        ElemOps on{};
-        if (have_n) {
-            int hid[L::HCH];
-            const int nh = staged_halo_ids<P>(a, tn.tile, hid);
-            staged_fill<P>(a, tn.tile, nh, hid, sU);
-            const int en = tn.tile * L::B + tid;
-            if (en < a.E) load_element_ops(a, a.nloc, en, on);
-        }
-        double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
-        if (live) element_update<P, NB_LDS_LEAN>(a, e, Uo, o.eg, W, dt, o.dtf, ti.st, ws_l, r0_l, r1_l);
-        if (a.stage == 2)
-            block_stage_max(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], tile_residual(a, ti));
+    TileInfo tn;
+    while (next_tile(a, it, nit, tn)) {  // uniform across the workgroup
+        ElemOpsRaw on;
+        pipe_tile<P, FLUX, true>(a, ti, o, sVe, sU, tn, on);
         ti = tn;
         o = on;
-        have = have_n;
     }
+    ElemOpsRaw unused;
+    pipe_tile<P, FLUX, false>(a, ti, o, sVe, sU, ti, unused);
 }
 
 // Register-pipelined persistent staged form (P1): the next tile's own modes, halo modes and
@@ -1224,12 +1326,12 @@ static void launch_stage_pf(int block, const StageArgs& a, hipStream_t s) {
         static bool opted = false;  // above 64 KiB the kernels must opt in
         if (lds > 65536 && !opted) {
             hipFuncSetAttribute((const void*)k_stage<P, FLUX, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipFuncSetAttribute((const void*)k_stage_pipe<P, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipFuncSetAttribute((const void*)k_stage_pipe<P <= 2 ? P : 2, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             opted = true;
         }
         if (P == 1 && a.pipe == 2 && !a.tile_mesh)  // + the LDS copy of 4 geometry rows
             hipLaunchKernelGGL((k_stage_rp<1, FLUX>), grid, blk, lds + 4 * staged_block(1) * sizeof(double), s, a);
-        else if (a.pipe) hipLaunchKernelGGL((k_stage_pipe<P, FLUX>), grid, blk, lds, s, a);
+        else if (P <= 2 && a.pipe) hipLaunchKernelGGL((k_stage_pipe<P <= 2 ? P : 2, FLUX>), grid, blk, lds, s, a);
         else hipLaunchKernelGGL((k_stage<P, FLUX, true>), grid, blk, lds, s, a);
         return;
     }

@@ -183,6 +183,40 @@ def test_p1_pipelined_tiles_match_oracle(monkeypatch):
     assert np.allclose(s.residual(), o.residual_change(), rtol=1e-9, atol=1e-15)
 
 
+@pytest.mark.parametrize("P,flux", [(2, "lf"), (2, "roe"), (1, "roe")])
+def test_pipelined_image_tiles_match_staged_and_oracle(monkeypatch, P, flux):
+    """RKDG_PIPE=1: persistent staged tiles whose next image is requested during the current
+    tile's update (step-start loads first, then a fixed count of direct loads).  529 tiles on
+    256 workgroups (RKDG_PERSIST=1) give each workgroup two or three tiles, so both the
+    prefetching body and the last-tile body run; against the one-tile-per-workgroup staged
+    kernel (same arithmetic, evaluated one edge point at a time) and the oracle."""
+    print(f"case: pipelined image tiles P{P} {flux}", flush=True)
+    g = Grid.splitquad(260, 260, 0.0, 0.0, 10.0, 10.0, periodic=True, diagonal=1, jitter=0.2, seed=22)
+    monkeypatch.setenv("RKDG_PIPE", "1")
+    monkeypatch.setenv("RKDG_PERSIST", "1")
+    o, s = make_pair(g, P=P, flux=flux, dt_policy="max", cfl=0.3)
+    monkeypatch.setenv("RKDG_PIPE", "0")
+    monkeypatch.delenv("RKDG_PERSIST")
+    r = Solver(g, degree=P, flux=flux, dt_policy="max", cfl=0.3)
+    o.project(1, vortex_params())
+    U0 = o.get_state()
+    s.set_state(U0)
+    r.set_state(U0)
+    s.advance(4)
+    r.advance(4)
+    o.advance(4)
+    Us, Ur = s.get_state(), r.get_state()
+    print("equal to the staged kernel bit for bit:", bool(np.array_equal(Us, Ur)), flush=True)
+    assert np.all(rel_l2(Us, Ur, o.mass()) < 1e-14)
+    err = rel_l2(Us, o.get_state(), o.mass())
+    assert np.all(err < TOL), err
+    (ts, ns), (tr, nr) = s.time(), r.time()
+    assert ns == nr == 4 and abs(ts - tr) <= 1e-13 * tr
+    assert np.allclose(s.residual(), o.residual_change(), rtol=1e-9, atol=1e-15)
+    s.close()
+    r.close()
+
+
 def test_timed_steps_equal_advance():
     """The measurement entry point rkdg_time_steps takes exactly the steps of rkdg_advance
     (graph-replayed pairs after an odd step count, device dt)."""
