@@ -1078,7 +1078,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 // tile's LDS image, passed as dynamic shared memory of 4 Np rows of staged_stride(P)
 // doubles; the workgroup then has staged_block(P) lanes).
 template <int P, int FLUX, bool STAGED>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED && P <= 2 ? 2 : 1))) k_stage(StageArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED && P == 1 && FLUX == RKDG_FLUX_ROE ? 3 : (STAGED && P <= 2 ? 2 : 1)))) k_stage(StageArgs a) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqe = D::Nqe;
     const DevTables& T = c_tab[P - 1];
@@ -1095,7 +1095,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         TileInfo ti;
         if (!tile_of_iter(a, it, nit, ti)) continue;  // uniform across the workgroup
-        stage_tile<P, FLUX, STAGED>(a, ti, sVe, sU, (int)threadIdx.x);
+        int tid = (int)threadIdx.x;
+        // P1 Roe staged tiles run three waves per SIMD: the lane index through an opaque copy
+        // per tile keeps the addresses derived from it inside the tile instead of live (spilled)
+        if constexpr (STAGED && P == 1 && FLUX == RKDG_FLUX_ROE) asm volatile("" : "+v"(tid));
+        stage_tile<P, FLUX, STAGED>(a, ti, sVe, sU, tid);
         __syncthreads();  // LDS and the reduction scratch are reused by the next tile
     }
 }
