@@ -179,12 +179,18 @@ __device__ __forceinline__ void roe_flux(double g, const double* UL, const doubl
 #pragma unroll
     for (int v = 0; v < 4; ++v) Fh[v] = FL[v] + FR[v];
     const double hl = (UL[3] + pl) * irl, hr = (UR[3] + pr) * irr;
-    const double sl = fsqrt(rl), sr = fsqrt(rr), isum = frcp(sl + sr);
-    const double rt = sl * sr;
-    const double ut = (sl * ul + sr * ur) * isum, vt = (sl * vl + sr * vr) * isum, ht = (sl * hl + sr * hr) * isum;
+    // Roe averages with the weight ratio s = sqrt(rho_R / rho_L): (a_L + s a_R) / (1 + s),
+    // and sqrt(rho_L rho_R) = rho_L s (one square root instead of two)
+    const double sw = fsqrt(rr * irl), isum = frcp(1.0 + sw);
+    const double rt = rl * sw;
+    const double ut = (ul + sw * ur) * isum, vt = (vl + sw * vr) * isum, ht = (hl + sw * hr) * isum;
     const double q2 = ut * ut + vt * vt;
     const double c2 = (g - 1) * (ht - 0.5 * q2);
-    const double ct = fsqrt(c2), ic2 = frcp(c2);
+    // c and 1/c^2 from one reciprocal square root (two Newton steps)
+    double ict = __builtin_amdgcn_rsq(c2);
+    ict = ict * fma(-0.5 * c2 * ict, ict, 1.5);
+    ict = ict * fma(-0.5 * c2 * ict, ict, 1.5);
+    const double ct = c2 * ict, ic2 = ict * ict;
     const double unt = ut * nx + vt * ny, utt = -ut * ny + vt * nx;
     const double dp = pr - pl, dr = rr - rl;
     const double unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
@@ -200,8 +206,8 @@ __device__ __forceinline__ void roe_flux(double g, const double* UL, const doubl
     const double l2 = fabs(unt);
     const double d1 = fmax(0.0, fmax((unt - ct) - (unl - cl), (unr - cr) - (unt - ct)));
     const double d4 = fmax(0.0, fmax((unt + ct) - (unl + cl), (unr + cr) - (unt + ct)));
-    if (l1 < d1) l1 = (l1 * l1 + d1 * d1) / (2 * d1);
-    if (l4 < d4) l4 = (l4 * l4 + d4 * d4) / (2 * d4);
+    if (l1 < d1) l1 = (l1 * l1 + d1 * d1) * (0.5 * frcp(d1));
+    if (l4 < d4) l4 = (l4 * l4 + d4 * d4) * (0.5 * frcp(d4));
     const double b1 = l1 * a1, b2 = l2 * a2, b3 = l2 * a3, b4 = l4 * a4;
     Fh[0] = 0.5 * (Fh[0] - (b1 + b2 + b4));
     Fh[1] = 0.5 * (Fh[1] - (b1 * (ut - ct * nx) + b2 * ut + b3 * (-ny) + b4 * (ut + ct * nx)));
