@@ -303,6 +303,11 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     c->ld = (long long)((c->E + 255) / 256) * 256;
     const int E = c->E;
     const long long ld = c->ld;
+    // the kernels address every array with 32-bit byte offsets
+    if (sizeof(double) * (size_t)4 * c->Np * ld >= (1ull << 32) || sizeof(double) * (size_t)GEO_ROWS * ld >= (1ull << 32)) {
+        delete c;
+        return fail(nullptr, RKDG_ERR_ARG, "mesh too large for one context (state or geometry array above 4 GiB)");
+    }
 
     // locality permutation
     c->order.resize(E);
@@ -469,12 +474,6 @@ int rkdg_create(const rkdg_mesh* m, const rkdg_params* prm, rkdg_ctx** out) {
     // the allocation-free step's scratch on this device (resolved once per device)
     c->scratch_rc = noalloc_scratch(c->dev, &c->scratch);
     size_t ubytes = sizeof(double) * (size_t)4 * c->Np * ld;
-    // the kernels address every array with 32-bit byte offsets
-    if (ubytes >= (1ull << 32) || sizeof(double) * (size_t)GEO_ROWS * ld >= (1ull << 32))
-    {
-        rkdg_destroy(c);
-        return fail(nullptr, RKDG_ERR_ARG, "mesh too large for one context (state or geometry array above 4 GiB)");
-    }
     for (int b = 0; b < 3; ++b) CREATECHK(hipMalloc(&c->U[b], ubytes));
     CREATECHK(hipMalloc(&c->geo, sizeof(double) * hgeo.size()));
     CREATECHK(hipMalloc(&c->nbr, sizeof(int) * hnbr.size()));

@@ -328,6 +328,27 @@ def test_create_rejects_an_empty_mesh():
     assert lib.rkdg_batch_create(0, None, None, ctypes.byref(h)) == L.ERR_ARG
 
 
+@pytest.mark.parametrize("degree,n,msg", [(1, (1 << 29), "too many elements"),
+                                           (3, 13_500_000, "above 4 GiB"),
+                                           (2, 22_500_000, "above 4 GiB")])
+def test_create_rejects_meshes_beyond_the_addressing_limits(degree, n, msg):
+    """Up to 2^29 - 1 elements (neighbour ids carry the edge index in two bits) and every
+    device array below 4 GiB (32-bit byte offsets): larger meshes are argument errors,
+    reported before any array is read or any device call (the arrays here hold 3 entries)."""
+    from rkdg_heter_calculate_amd.solver import make_params
+    lib = L.load()
+    d = (ctypes.c_double * 3)()
+    i = (ctypes.c_int32 * 9)()
+    m = L.Mesh()
+    m.n_vertices, m.n_elements = 3, n
+    m.vx = m.vy = ctypes.cast(d, L.P_double)
+    m.tri = m.neighbour = ctypes.cast(i, L.P_int32)
+    prm = make_params(degree=degree)
+    h = ctypes.c_void_p()
+    assert lib.rkdg_create(ctypes.byref(m), ctypes.byref(prm), ctypes.byref(h)) == L.ERR_ARG
+    assert msg in lib.rkdg_last_error(None).decode()
+
+
 def _batch_create_each(grids, params):
     """rkdg_batch_create with one parameter set per mesh; (status, message)."""
     from rkdg_heter_calculate_amd.solver import make_params, mesh_view
