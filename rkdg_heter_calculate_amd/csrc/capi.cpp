@@ -1014,20 +1014,23 @@ int rkdg_measure_copy(int32_t device, int64_t bytes, int32_t reps, double* gbs) 
         hipFree(a); hipFree(b); hipStreamDestroy(s);
         return fail(nullptr, RKDG_ERR_NOMEM, "measure_copy buffers");
     }
-    hipMemsetAsync(a, 0, bytes, s);
-    hipEvent_t e0, e9;
-    hipEventCreate(&e0);
-    hipEventCreate(&e9);
-    launch_stream_copy(a, b, bytes, s);  // warm-up
-    hipEventRecord(e0, s);
-    for (int r = 0; r < reps; ++r) launch_stream_copy(a, b, bytes, s);
-    hipEventRecord(e9, s);
-    hipError_t st = hipEventSynchronize(e9);
+    // the first failing call decides the status; cleanup is best effort
+    hipError_t st = hipMemsetAsync(a, 0, bytes, s);
+    hipEvent_t e0 = nullptr, e9 = nullptr;
+    if (st == hipSuccess) st = hipEventCreate(&e0);
+    if (st == hipSuccess) st = hipEventCreate(&e9);
+    if (st == hipSuccess) st = (hipError_t)launch_stream_copy(a, b, bytes, s);  // warm-up
+    if (st == hipSuccess) st = hipEventRecord(e0, s);
+    for (int r = 0; r < reps && st == hipSuccess; ++r) st = (hipError_t)launch_stream_copy(a, b, bytes, s);
+    if (st == hipSuccess) st = hipEventRecord(e9, s);
+    if (st == hipSuccess) st = hipEventSynchronize(e9);
     float ms = 0;
-    hipEventElapsedTime(&ms, e0, e9);
-    hipEventDestroy(e0); hipEventDestroy(e9);
-    hipFree(a); hipFree(b); hipStreamDestroy(s);
+    if (st == hipSuccess) st = hipEventElapsedTime(&ms, e0, e9);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e9) (void)hipEventDestroy(e9);
+    (void)hipFree(a); (void)hipFree(b); (void)hipStreamDestroy(s);
     if (st != hipSuccess) return fail(nullptr, RKDG_ERR_HIP, hipGetErrorString(st));
+    if (!(ms > 0)) return fail(nullptr, RKDG_ERR_HIP, "measure_copy: no elapsed time");
     *gbs = 2.0 * (double)bytes * reps / (ms / 1e3) / 1e9;
     return RKDG_OK;
 }

@@ -369,7 +369,7 @@ __device__ __forceinline__ void element_residual(const StageArgs& a, int e, cons
                                                  const ElemGeo& eg, int w0, int w1, int w2, const double* sVe,
                                                  const double* sU, ElemWork<P>& W, const double* sGe = nullptr) {
     using D = Deg<P>;
-    constexpr int Np = D::Np, Nqv = D::Nqv, Nqe = D::Nqe;
+    constexpr int Np = D::Np, Nqe = D::Nqe;
     constexpr bool PF = NBSRC == NB_GLOBAL && Prefetch<P>::on;
     const DevTables& T = c_tab[P - 1];
     const double g = a.gamma;
@@ -1335,9 +1335,9 @@ static void launch_stage_pf(int block, const StageArgs& a, hipStream_t s) {
         constexpr size_t lds = (size_t)4 * Deg<P>::Np * staged_stride(P) * sizeof(double);
         static bool opted = false;  // above 64 KiB the kernels must opt in
         if (lds > 65536 && !opted) {
-            hipFuncSetAttribute((const void*)k_stage<P, FLUX, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipFuncSetAttribute((const void*)k_stage_pipe<P <= 2 ? P : 2, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            opted = true;
+            // (a failure leaves the attribute unset: the launch then fails and reports it)
+            opted = hipFuncSetAttribute((const void*)k_stage<P, FLUX, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess &&
+                    hipFuncSetAttribute((const void*)k_stage_pipe<P <= 2 ? P : 2, FLUX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
         }
         if (P == 1 && a.pipe == 2 && !a.tile_mesh)  // + the LDS copy of 4 geometry rows
             hipLaunchKernelGGL((k_stage_rp<1, FLUX>), grid, blk, lds + 4 * staged_block(1) * sizeof(double), s, a);
@@ -1366,8 +1366,7 @@ static void launch_stage_px_pf(const StageArgs& a, hipStream_t s) {
     constexpr size_t lds = (size_t)4 * Deg<P>::Np * staged_stride(P) * sizeof(double);
     static bool opted = false;  // above 64 KiB the kernel must opt in
     if (lds > 65536 && !opted) {
-        hipFuncSetAttribute((const void*)k_stage_px<P, FLUX, CT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        opted = true;
+        opted = hipFuncSetAttribute((const void*)k_stage_px<P, FLUX, CT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
     }
     hipLaunchKernelGGL((k_stage_px<P, FLUX, CT>), dim3(grid_blocks(a)), dim3(staged_block(P)), lds, s, a);
 }
@@ -1382,8 +1381,7 @@ template <int FLUX, int CT>
 static void launch_stage_pz_f(const StageArgs& a, hipStream_t s) {
     static bool opted = false;  // above 64 KiB the kernel must opt in
     if (!opted) {
-        hipFuncSetAttribute((const void*)k_stage_pz<3, FLUX, CT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Pz<3>::LDS);
-        opted = true;
+        opted = hipFuncSetAttribute((const void*)k_stage_pz<3, FLUX, CT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Pz<3>::LDS) == hipSuccess;
     }
     hipLaunchKernelGGL((k_stage_pz<3, FLUX, CT>), dim3(grid_blocks(a)), dim3(Pz<3>::B), Pz<3>::LDS, s, a);
 }
