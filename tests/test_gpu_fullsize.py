@@ -6,14 +6,16 @@
 * the 1024x1024 four-quadrant Riemann mesh (seed 2025, 2,097,152 triangles), P1, Roe,
   TVB M = 50 with positivity scaling, 3 steps (1e-12, equal limiter flags);
 * the same problem shrunk to 64x64 and run to t = 0.3 (about 3,500 steps), where the
-  limiter's decisions could drift apart: relative L2 and the fraction of equal flags.
+  limiter's decisions could drift apart: relative L2 and the fraction of equal flags;
+* the P3 Roe vortex mesh family at 1024x1024 (seed 2026, 2,097,152 triangles), 2 steps, in the
+  FP64 form (1e-12) and the precision form's FP32 (1e-6) and BF16 (2e-4) contractions.
 """
 import numpy as np
 import pytest
 
 from oracle import pyoracle as po
 from rkdg_heter_calculate_amd import IC_RIEMANN4, IC_VORTEX, Solver, vortex_params
-from rkdg_heter_calculate_amd.workloads import riemann_p1, vortex_grid
+from rkdg_heter_calculate_amd.workloads import riemann_p1, vortex_grid, vortex_p3
 from helpers import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -51,6 +53,36 @@ def test_vortex_p2_512_matches_oracle(prec, tol):
     err = rel_l2(Us, Uo, M)
     linf = [float(np.max(np.abs(Us[v] - Uo[v])) / np.max(np.abs(Uo[v]))) for v in range(4)]
     print("relative L2 per variable", err, "relative Linf", linf, flush=True)
+    assert np.all(err < tol), err
+    s.close()
+
+
+def vortex_p3_1024_reference():
+    if "p3" not in _REF:
+        g, kw, vp, _ = vortex_p3(1024)  # the P3 workload's mesh family at 2.1M triangles
+        o = oracle_for(g, P=3, flux=1, dt_policy=1, cfl=kw["cfl"])
+        o.project(1, vp)
+        U0 = o.get_state()
+        n, t, _ = o.advance(2)
+        _REF["p3"] = (g, kw, U0, o.get_state(), t, o.mass())
+    return _REF["p3"]
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-6), ("bf16", 2e-4)])
+def test_vortex_p3_1024_matches_oracle(prec, tol):
+    """P3 Roe on 2,097,152 triangles, 2 steps: the FP64 global-gather kernel (64-element
+    tiles) and the precision form's FP32-image tiles (256 elements, halo capacity checked per
+    tile) against the oracle."""
+    print(f"case: 1024x1024 P3 Roe vortex 2 steps {prec}", flush=True)
+    g, kw, U0, Uo, to, M = vortex_p3_1024_reference()
+    assert g.n_elements == 2097152
+    s = Solver(g, precision=prec, **kw)
+    s.set_state(U0)
+    n, t = s.advance(2)
+    assert n == 2 and abs(t - to) <= (1e-13 if prec == "fp64" else tol) * to
+    Us = s.get_state()
+    err = rel_l2(Us, Uo, M)
+    print("relative L2 per variable", err, flush=True)
     assert np.all(err < tol), err
     s.close()
 
