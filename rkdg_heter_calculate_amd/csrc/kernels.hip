@@ -767,7 +767,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileInfo& t
     }
 
     if (e < a.E)
-        element_core<P, FLUX, STAGED ? NB_LDS : NB_GLOBAL>(a, e, Uo, o.eg, o.w0, o.w1, o.w2, dt, o.dtf, sVe, sU,
+        element_core<P, FLUX, STAGED ? (P == 1 && FLUX == RKDG_FLUX_LF ? NB_LDS_LEAN : NB_LDS) : NB_GLOBAL>(a, e, Uo, o.eg, o.w0, o.w1, o.w2, dt, o.dtf, sVe, sU,
                                                           ti.st, ws_l, r0_l, r1_l);
     if (a.stage == 2)
         block_stage_max(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], tile_residual(a, ti));
@@ -1084,7 +1084,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 // tile's LDS image, passed as dynamic shared memory of 4 Np rows of staged_stride(P)
 // doubles; the workgroup then has staged_block(P) lanes).
 template <int P, int FLUX, bool STAGED>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED && P == 1 && FLUX == RKDG_FLUX_ROE ? 3 : (STAGED && P <= 2 ? 2 : 1)))) k_stage(StageArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED && P == 1 ? (FLUX == RKDG_FLUX_LF ? 4 : 3) : (STAGED && P <= 2 ? 2 : 1)))) k_stage(StageArgs a) {
     using D = Deg<P>;
     constexpr int Np = D::Np, Nqe = D::Nqe;
     const DevTables& T = c_tab[P - 1];
@@ -1102,9 +1102,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGED
         TileInfo ti;
         if (!tile_of_iter(a, it, nit, ti)) continue;  // uniform across the workgroup
         int tid = (int)threadIdx.x;
-        // P1 Roe staged tiles run three waves per SIMD: the lane index through an opaque copy
-        // per tile keeps the addresses derived from it inside the tile instead of live (spilled)
-        if constexpr (STAGED && P == 1 && FLUX == RKDG_FLUX_ROE) asm volatile("" : "+v"(tid));
+        // P1 staged tiles run four (LF, one edge point at a time) or three (Roe) waves per
+        // SIMD: the lane index through an opaque copy per tile keeps the addresses derived
+        // from it inside the tile instead of live (spilled)
+        if constexpr (STAGED && P == 1) asm volatile("" : "+v"(tid));
         stage_tile<P, FLUX, STAGED>(a, ti, sVe, sU, tid);
         __syncthreads();  // LDS and the reduction scratch are reused by the next tile
     }
