@@ -902,7 +902,9 @@ __device__ __forceinline__ void pipe_tile(const StageArgs& a, const TileInfo& ti
     if constexpr (PF) staged_fill_fixed<P>(a, tn.tile, hoff, sU);
     double ws_l = 0.0, r0_l = 0.0, r1_l = 0.0;
     if (live) element_update<P, NB_LDS, true>(a, e, Uo, o.eg, W, dt, dtf, ti.st, ws_l, r0_l, r1_l);
-    if (a.stage == 2)
+    // (past the end time the tile keeps its last residual pair, as the staged kernel does;
+    // `done` is the same for every workgroup of the mesh)
+    if (a.stage == 2 && !done)
         block_stage_max(ws_l, r0_l, r1_l, &ti.st->ws[1 - a.slot], tile_residual(a, ti));
 }
 
