@@ -1,0 +1,44 @@
+"""Conservation on periodic meshes, every form: the totals sum_e A_e U_e (mode 0 of the element
+basis is the cell mean) after 100 steps against the start.  Both elements of an edge compute
+its flux, each with the normal of its own inverse Jacobian, so the two contributions cancel
+to the rounding of each form: FP64 arithmetic in the FP64 form; in the precision forms the
+flux deviations F^_j - C are rounded (FP32 or BF16) on each side, while the fluxes of the two
+cell means, C, enter both sides in FP64.  The bound per form is stated in the test and scaled
+by sum_e A_e |U_e|.
+"""
+import numpy as np
+import pytest
+
+from rkdg_heter_calculate_amd import IC_VORTEX, Solver, vortex_params
+from helpers import vortex_grid
+
+pytestmark = pytest.mark.gpu
+
+BOUND = {"fp64": 1e-13, "fp32": 2e-9, "bf16": 1e-5}  # measured: 6e-15, 6e-10, 3e-6
+
+
+def areas(g):
+    ex, ey = g.element_coords()
+    return 0.5 * np.abs((ex[:, 1] - ex[:, 0]) * (ey[:, 2] - ey[:, 0]) - (ex[:, 2] - ex[:, 0]) * (ey[:, 1] - ey[:, 0]))
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+@pytest.mark.parametrize("flux", ["lf", "roe"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+def test_periodic_totals_are_conserved(prec, flux, P):
+    print(f"case: conservation {prec} {flux} P{P}", flush=True)
+    g = vortex_grid(24, jitter=0.2, diagonal=1, seed=9)
+    A = areas(g)
+    s = Solver(g, degree=P, flux=flux, dt_policy="max", cfl=0.3 if P < 3 else 0.15, precision=prec)
+    s.project_ic(IC_VORTEX, vortex_params())
+    U0 = s.get_state()
+    tot0 = (A * U0[:, 0, :]).sum(axis=1)
+    scale = (A * np.abs(U0[:, 0, :])).sum(axis=1)
+    s.advance(100)
+    U = s.get_state()
+    drift = np.abs((A * U[:, 0, :]).sum(axis=1) - tot0) / scale
+    print("relative drift of the totals per variable", drift, flush=True)
+    assert np.all(drift < BOUND[prec]), drift
+    # the state really moved (the vortex is advected through the periodic box)
+    assert np.max(np.abs(U[0] - U0[0])) > 1e-3
+    s.close()
