@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from rkdg_heter_calculate_amd import IC_VORTEX, Solver, vortex_params
+from rkdg_heter_calculate_amd._lib import BASIS_NATIVE
 from helpers import vortex_grid
 
 pytestmark = pytest.mark.gpu
@@ -41,4 +42,35 @@ def test_periodic_totals_are_conserved(prec, flux, P):
     assert np.all(drift < BOUND[prec]), drift
     # the state really moved (the vortex is advected through the periodic box)
     assert np.max(np.abs(U[0] - U0[0])) > 1e-3
+    s.close()
+
+
+# Free stream, in the library's own basis (psi_0 = 1: a uniform state is its mode 0 alone,
+# exactly; the reference's per-element basis would add the rounding of the change of basis,
+# scaled up by its cubic modes' small norms).  The FP64 form's residual of a uniform state is
+# F . (A (xi Gxi + eta Geta) - sum_k l_k/2 n_k Se_k) per mode: zero by the divergence theorem
+# but for the rounding of the tables (measured after 20 steps: at most 1.2e-14, P1-P3, and the
+# P1/P2 precision forms, which add that part in FP64, alike).  The P3 precision form takes its
+# deviations from fluxes of the mean that cancel that sum exactly (measured: 0 in every mode);
+# what could remain is the rounding of the SSP-RK3 combinations of the means.
+FREE_STREAM_BOUND = 1e-13
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+@pytest.mark.parametrize("flux", ["lf", "roe"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+def test_free_stream_is_preserved(prec, flux, P):
+    """A uniform state stays uniform on a jittered periodic mesh.  Its point values equal its
+    means, so in the precision forms every deviation and flux deviation is exactly zero."""
+    print(f"case: free stream {prec} {flux} P{P}", flush=True)
+    g = vortex_grid(24, jitter=0.2, diagonal=1, seed=10)
+    s = Solver(g, degree=P, flux=flux, dt_policy="max", cfl=0.3 if P < 3 else 0.15, precision=prec)
+    U0 = np.zeros(s.state_shape())
+    U0[:, 0, :] = np.array([1.2, 0.4, -0.3, 2.9])[:, None]
+    s.set_state(U0, basis=BASIS_NATIVE)
+    s.advance(20)
+    U = s.get_state(basis=BASIS_NATIVE)
+    dev = np.abs(U - U0).max(axis=(0, 2))
+    print("max deviation from the free stream per mode", dev, flush=True)
+    assert dev.max() < FREE_STREAM_BOUND, dev
     s.close()
