@@ -61,10 +61,14 @@ def test_jittered_steps(P):
     assert np.all(err < TOL), err
 
 
-@pytest.mark.parametrize("P,prec", [(2, "fp64"), (1, "fp64"), (3, "fp64"), (2, "fp32"), (3, "fp32")])
+@pytest.mark.parametrize("P,prec", [(2, "fp64"), (1, "fp64"), (3, "fp64"), (2, "fp32"), (3, "fp32"), (2, "bf16"),
+                                    (3, "bf16")])
 def test_boundaries_match_oracle(P, prec):
     """Far-field, wall and transmissive edges (per-edge treatment of the ghost kernel), in the
-    FP64 kernels and in the precision kernels (one partial 256-element tile)."""
+    FP64 kernels and in the precision kernels (one partial 256-element tile).  BF16: 1e-3
+    relative L2 per variable (measured 0.6-4.4e-4; the free stream's rho v is a fifth of its
+    rho u, so the same absolute error, 0.4-2.6e-4 over the four variables, is a larger
+    fraction of it than on the vortex workloads' 2e-4 bar), the residual of the means to 2e-2."""
     print(f"case: boundaries P{P} {prec}", flush=True)
     g = Grid.splitquad(12, 10, 0.0, 0.0, 10.0, 8.0, periodic=False, diagonal=1, jitter=0.15, seed=5)
     bnd = g.boundary.copy()
@@ -80,8 +84,8 @@ def test_boundaries_match_oracle(P, prec):
     fs = (1.0, 0.5, 0.1, 2.63)
     for flux in ("lf", "roe"):
         o, s = make_pair(g, P=P, flux=flux, dt_policy="max", cfl=0.2 if P < 3 else 0.1, freestream=fs)
-        if prec == "fp32":
-            s = Solver(g, degree=P, flux=flux, dt_policy="max", cfl=0.2 if P < 3 else 0.1, precision="fp32",
+        if prec != "fp64":
+            s = Solver(g, degree=P, flux=flux, dt_policy="max", cfl=0.2 if P < 3 else 0.1, precision=prec,
                        freestream=fs)
         vp = vortex_params(uinf=0.5, vinf=0.1, xc=4.0, yc=4.0, lx=0.0, ly=0.0)
         o.project(1, vp)
@@ -89,9 +93,12 @@ def test_boundaries_match_oracle(P, prec):
         s.advance(8)
         o.advance(8)
         err = rel_l2(s.get_state(), o.get_state(), o.mass())
-        assert np.all(err < (TOL if prec == "fp64" else MX_TOL)), (flux, err)
+        print(f"{flux}: relative L2 per variable {err}", flush=True)
+        assert np.all(err < {"fp64": TOL, "fp32": MX_TOL, "bf16": 1e-3}[prec]), (flux, err)
+        print("residual of the means, GPU and oracle:", s.residual(), o.residual_change(), flush=True)
         # the reference's residual (max change of the rho and rhoE means), reduced over tiles
-        assert np.allclose(s.residual(), o.residual_change(), rtol=1e-9 if prec == "fp64" else 1e-3, atol=1e-15)
+        assert np.allclose(s.residual(), o.residual_change(), rtol={"fp64": 1e-9, "fp32": 1e-3, "bf16": 2e-2}[prec],
+                           atol=1e-15)
 
 
 def test_limiter_matches_oracle():
