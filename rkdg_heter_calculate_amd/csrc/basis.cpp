@@ -173,22 +173,6 @@ void build_ref_basis(int P, RefBasis& B) {
             T.Se[k][i] = se;
         }
     }
-    // single-precision operators of the P1/P2 precision kernels: rows = evaluation points
-    // (volume then edges: 7 + 3 x 3 = 16; P3's 12 + 3 x 4 points have their own tables)
-    if (nq + 3 * ne <= 16) {
-        for (int q = 0; q < nq; ++q)
-            for (int i = 0; i < Np; ++i) T.fVv[q][i] = (float)T.Vv[q][i];
-        for (int k = 0; k < 3; ++k)
-            for (int j = 0; j < ne; ++j)
-                for (int i = 0; i < Np; ++i) T.fVv[nq + k * ne + j][i] = (float)T.Ve[k][j][i];
-    }
-    for (int q = 0; q < nq; ++q) {
-        T.fwv[q] = (float)T.wv[q];
-        for (int i = 0; i < Np; ++i) {
-            T.fDxi[q][i] = (float)T.Dxi[q][i];
-            T.fDeta[q][i] = (float)T.Deta[q][i];
-        }
-    }
 }
 
 }  // namespace

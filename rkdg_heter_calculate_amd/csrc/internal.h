@@ -63,16 +63,12 @@ struct DevTables {
     double minv[MAXNP];            // 1 / (quadrature mass per unit area)
     double Vmid[3][3];             // P1 modes at the edge midpoints (limiter)
     double recon[2][3];            // modes 1,2 from midpoint deviations (limiter)
-    // the volume and edge rules applied to a constant flux (the precision form's mean-state
-    // part, in FP64): sum_q w_q d psi_i / d xi (eta), and sum_j we_j psi_i(edge k point j)
+    // the volume and edge rules applied to a constant flux (the precision forms' mean-state
+    // part: in FP64 at P1/P2; Se in the edge corrections at P3): sum_q w_q d psi_i / d xi
+    // (eta), and sum_j we_j psi_i(edge k point j)
     double Gxi[MAXNP];
     double Geta[MAXNP];
     double Se[3][MAXNP];
-    // single-precision copies for the MFMA form
-    float fVv[16][16];             // P1/P2: [point (vol 0..Nqv-1, then 3*Nqe edge)][mode], zero padded
-    float fDxi[MAXQV][MAXNP];      // single-precision gradients at the volume points
-    float fDeta[MAXQV][MAXNP];
-    float fwv[MAXQV];
 };
 
 // Device-side step state: wave-speed maxima and times per step parity.

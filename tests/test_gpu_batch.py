@@ -15,6 +15,7 @@ import pytest
 
 from oracle import pyoracle as po
 from rkdg_heter_calculate_amd import BC_FARFIELD, Batch, Grid
+from rkdg_heter_calculate_amd._lib import RkdgError
 from rkdg_heter_calculate_amd.workloads import batch_instances, vortex_grid
 from helpers import rel_l2
 
@@ -136,3 +137,11 @@ def test_farfield_instances_match_oracle():
     ref = [oracle_run(g, 2, vp, c, fs=f) for (g, vp, c), f in zip(inst, fss)]
     check_against_oracle(b, inst, ref, "fp64", "far-field batch")
     b.close()
+
+
+def test_bf16_batch_at_p3_is_rejected():
+    """BF16 contractions in a batch run on the staged lane-per-element tiles of P1/P2; a P3
+    batch (four lanes per element, FP32 only) refuses BF16 with an error, not a silent FP32."""
+    inst, _ = oracle_states(3)
+    with pytest.raises(RkdgError, match="BF16"):
+        make_batch(inst[:2], 3, "bf16")
