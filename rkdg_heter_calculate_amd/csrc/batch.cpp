@@ -105,7 +105,7 @@ static StageArgs bargs(rkdg_batch* b, int stage, int slot) {
     a.hoff = b->hoff;
     a.hids = b->hids;
     a.staged = b->staged;
-    a.lds_stride = staged_stride(b->P);
+    a.lds_stride = b->prm.precision != RKDG_PREC_FP64 ? px_stride(b->P) : staged_stride(b->P);
     a.dtflag = b->dtflag;
     a.st = b->st;
     a.tile_mesh = b->tile_mesh;
@@ -155,8 +155,6 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     if (p0.limiter != RKDG_LIMITER_NONE) return bfail(nullptr, RKDG_ERR_ARG, "the batching form runs without limiter");
     if (p0.precision != RKDG_PREC_FP64 && p0.precision != RKDG_PREC_FP32_MFMA && p0.precision != RKDG_PREC_BF16_MFMA)
         return bfail(nullptr, RKDG_ERR_ARG, "precision: RKDG_PREC_FP64, RKDG_PREC_FP32_MFMA or RKDG_PREC_BF16_MFMA");
-    if (p0.precision == RKDG_PREC_BF16_MFMA && p0.degree > 2)
-        return bfail(nullptr, RKDG_ERR_ARG, "RKDG_PREC_BF16_MFMA in a batch: degree 1 or 2 (the staged tiles)");
     if (p0.block != 0 && (p0.block < 64 || p0.block > 256 || p0.block % 64 != 0 || 256 % p0.block != 0))
         return bfail(nullptr, RKDG_ERR_ARG, "block: 0 (library default) or 64, 128, 256");
     // The instances share one launch, so everything but the mesh, the initial state and
@@ -344,13 +342,17 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
     BCREATECHK(hipMemcpy(b->dtflag, hflag.data(), hflag.size(), hipMemcpyHostToDevice));
     BCREATECHK(hipMemcpy(b->cfl, hcfl.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     b->bytes = 3 * ub + sizeof(double) * hgeo.size() + sizeof(int) * hnbr.size() + hflag.size();
-    if (b->P <= 2) {
+    const bool prec_form = p0.precision != RKDG_PREC_FP64;
+    if (b->P <= 2 || prec_form) {
         // staged-form tables for 256-element tiles (instances start on 256-element
-        // boundaries, so no tile spans two instances)
+        // boundaries, so no tile spans two instances): the FP64 form's at P1/P2, the
+        // precision form's at every degree (at P3 its FP32-image tiles, k_stage_pz)
         std::vector<int> hoff, hids, hnloc;
-        const int B = staged_block(b->P);
+        const int B = prec_form ? px_block(b->P) : staged_block(b->P);
+        const int stride = prec_form ? px_stride(b->P) : staged_stride(b->P);
+        const long long lds = prec_form ? px_lds_bytes(b->P) : 4LL * b->Np * stride * (long long)sizeof(double);
         const int hmax = build_halo(hnbr.data(), ld, (int)ld, B, hoff, hids, hnloc);
-        if (B + hmax <= staged_stride(b->P)) {
+        if (B + hmax <= stride && lds <= max_lds_per_block(b->dev)) {
             BCREATECHK(hipMalloc(&b->nloc, sizeof(int) * hnloc.size()));
             BCREATECHK(hipMalloc(&b->hoff, sizeof(int) * hoff.size()));
             BCREATECHK(hipMalloc(&b->hids, sizeof(int) * std::max<size_t>(1, hids.size())));
@@ -369,8 +371,8 @@ int rkdg_batch_create(int32_t n, const rkdg_mesh* meshes, const rkdg_params* par
         rkdg_batch_destroy(b);
         return bfail(nullptr, RKDG_ERR_ARG, "RKDG_PREC_BF16_MFMA: a tile's halo does not fit the staged LDS image");
     }
-    if (p0.precision != RKDG_PREC_FP64 && b->staged_ok) {
-        b->block = staged_block(b->P);
+    if (prec_form && b->staged_ok) {
+        b->block = px_block(b->P);
         b->staged = 1;
     }
     // FP64: the global-gather tiles until rkdg_batch_autotune picks; RKDG_BATCH_STAGED=1
@@ -561,7 +563,10 @@ static int device_cus(int dev) {
 // one workgroup per tile, else a persistent grid of per_cu 256-thread workgroups per CU.
 static bool tiles_valid(const rkdg_batch* b, int staged, int block, int per_cu) {
     if (per_cu != 0 && per_cu != 4 && per_cu != 8) return false;
-    if (staged) return b->staged_ok && b->P <= 2 && block == staged_block(b->P);
+    if (staged) {
+        if (b->prm.precision != RKDG_PREC_FP64) return b->staged_ok && block == px_block(b->P);
+        return b->staged_ok && b->P <= 2 && block == staged_block(b->P);
+    }
     if (b->prm.precision == RKDG_PREC_BF16_MFMA) return false;  // staged tiles only
     if (b->prm.precision == RKDG_PREC_FP32_MFMA) return block == 64;
     return block == 64 || block == 128 || block == 256;

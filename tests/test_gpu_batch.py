@@ -15,7 +15,6 @@ import pytest
 
 from oracle import pyoracle as po
 from rkdg_heter_calculate_amd import BC_FARFIELD, Batch, Grid
-from rkdg_heter_calculate_amd._lib import RkdgError
 from rkdg_heter_calculate_amd.workloads import batch_instances, vortex_grid
 from helpers import rel_l2
 
@@ -82,7 +81,7 @@ def test_instance_set_spans_tile_counts():
 
 
 @pytest.mark.parametrize("P,prec", [(2, "fp64"), (2, "fp32"), (1, "fp64"), (3, "fp64"), (1, "fp32"), (3, "fp32"),
-                                    (2, "bf16"), (1, "bf16")])
+                                    (2, "bf16"), (1, "bf16"), (3, "bf16")])
 def test_autotuned_batch_matches_oracle(P, prec):
     print(f"case: autotuned batch P{P} {prec}", flush=True)
     inst, ref = oracle_states(P)
@@ -139,9 +138,16 @@ def test_farfield_instances_match_oracle():
     b.close()
 
 
-def test_bf16_batch_at_p3_is_rejected():
-    """BF16 contractions in a batch run on the staged lane-per-element tiles of P1/P2; a P3
-    batch (four lanes per element, FP32 only) refuses BF16 with an error, not a silent FP32."""
-    inst, _ = oracle_states(3)
-    with pytest.raises(RkdgError, match="BF16"):
-        make_batch(inst[:2], 3, "bf16")
+@pytest.mark.parametrize("prec,per_cu", [("fp32", 0), ("fp32", 4), ("bf16", 0), ("bf16", 8)])
+def test_p3_precision_batch_runs_on_staged_tiles(prec, per_cu):
+    """At P3 the precision form's batch runs on the 256-element FP32-image tiles (k_stage_pz),
+    one workgroup per tile or a persistent grid, and matches the oracle within the form's bar."""
+    print(f"case: batch P3 {prec} staged tiles per_cu={per_cu}", flush=True)
+    inst, ref = oracle_states(3)
+    b = make_batch(inst, 3, prec)
+    assert b.tiles()["staged"] == 1 and b.tiles()["block"] == 256
+    b.set_tiles(1, 256, per_cu)
+    b.advance(STEPS)
+    b.sync()
+    check_against_oracle(b, inst, ref, prec, f"P3 {prec} staged per_cu={per_cu}")
+    b.close()
