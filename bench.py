@@ -64,8 +64,9 @@ def large_p2_point(n, steps):
     return out
 
 
-def batch_point(count, steps, precision="fp64"):
-    """Batched sweep: `count` periodic vortex instances of uneven size in one launch per stage."""
+def batch_point(count, steps, precision="fp64", degree=2):
+    """Batched sweep: `count` periodic vortex instances of uneven size in one launch per stage
+    (at P3 with half the instances' CFL numbers)."""
     import ctypes
     import numpy as np
     from rkdg_heter_calculate_amd import _lib as L
@@ -79,7 +80,8 @@ def batch_point(count, steps, precision="fp64"):
     params = (L.Params * count)()
     for i in range(count):
         meshes[i] = views[i]
-        params[i] = make_params(degree=2, flux="lf", dt_policy="max", cfl=inst[i][2], precision=precision)
+        params[i] = make_params(degree=degree, flux="lf", dt_policy="max",
+                                cfl=inst[i][2] * (0.5 if degree == 3 else 1.0), precision=precision)
     h = ctypes.c_void_p()
     L.bcheck(lib.rkdg_batch_create(count, meshes, params, ctypes.byref(h)), None, "batch_create")
     for i in range(count):
@@ -97,8 +99,9 @@ def batch_point(count, steps, precision="fp64"):
     lib.rkdg_batch_info(h, info.ctypes.data_as(L.P_int64))
     lib.rkdg_batch_destroy(h)
     E = int(info[1])
-    return {"instances": count, "elements": E, "steps": steps, "block": int(best.value),
-            "value": E * 6 * 4 * 3 * steps / dt, "ms_per_step": dt * 1e3 / steps,
+    Np = (degree + 1) * (degree + 2) // 2
+    return {"instances": count, "elements": E, "degree": degree, "steps": steps, "block": int(best.value),
+            "value": E * Np * 4 * 3 * steps / dt, "ms_per_step": dt * 1e3 / steps,
             "note": f"{precision} batched form, hipGraph-replayed stages, autotuned tiles and persistence"}
 
 
@@ -306,7 +309,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip every extra leg")
     ap.add_argument("--extra", default="p2_2M,batch,batch_fp32",
-                    help="comma-separated extra legs: p2_2M, batch, batch_fp32, batch_bf16, riemann, p3 "
+                    help="comma-separated extra legs: p2_2M, batch, batch_fp32, batch_bf16, batch_p3, riemann, p3 "
                          "(riemann runs 2 x ~25k steps to t = 0.3 and p3 builds an 8.4M-triangle mesh)")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-MFMA leg")
     ap.add_argument("--form", choices=("fp64", "fp32", "bf16"), default="fp64",
@@ -402,7 +405,9 @@ def main():
                    "batch_fp32": ("batch_fp32_mfma", lambda: batch_point(args.batch, 100, precision="fp32")),
                    "batch_bf16": ("batch_bf16_mfma", lambda: batch_point(args.batch, 100, precision="bf16")),
                    "riemann": ("riemann_p1_2M", lambda: riemann_point(1024, args.riemann_t_end)),
-                   "p3": ("vortex_p3_8M", lambda: p3_point(2048, 50))}
+                   "p3": ("vortex_p3_8M", lambda: p3_point(2048, 50)),
+                   "batch_p3": ("batch_p3", lambda: {p: batch_point(args.batch, 50, precision=p, degree=3)
+                                                     for p in ("fp64", "fp32", "bf16")})}
         for leg in legs:
             if leg not in runners:
                 sys.exit(f"bench.py: unknown extra leg {leg!r}")
