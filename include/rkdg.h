@@ -218,9 +218,10 @@ int rkdg_batch_sync(rkdg_batch* b);
 /* Autotuner: times every tile configuration the batch admits (rkdg_batch_set_tiles) on a
  * repeatable stage and keeps the fastest; best_block may be NULL. */
 int rkdg_batch_autotune(rkdg_batch* b, int32_t* best_block);
-/* Selects one tile configuration: staged != 0 for the staged 256-element tiles (P1/P2, when
- * every tile's halo fits its LDS image), else global-gather tiles of block = 64, 128 or 256
- * elements (the FP32-MFMA form: 64); per_cu 0 launches one workgroup per tile, 4 or 8 a
+/* Selects one tile configuration: staged != 0 for the staged 256-element tiles (FP64 form:
+ * P1/P2; precision forms: P1-P3; when every tile's halo fits its LDS image), else
+ * global-gather tiles of block = 64, 128 or 256 elements (the FP32-MFMA form: 64; the
+ * BF16-MFMA form has staged tiles only); per_cu 0 launches one workgroup per tile, 4 or 8 a
  * persistent grid of that many 256-thread workgroups per CU.  RKDG_ERR_ARG when the batch
  * does not admit it.  rkdg_batch_get_tiles reports the configuration in use as
  * {staged, block, per_cu}. */
