@@ -67,11 +67,12 @@ def test_noalloc_step_equals_stages_then_a_smaller_mesh():
     res.free()
 
 
-@pytest.mark.parametrize("P,flux", [(1, "lf"), (3, "roe"), (2, "roe")])
-def test_noalloc_two_steps_on_one_context(P, flux):
-    print(f"case: step_noalloc two steps P{P} {flux}", flush=True)
+@pytest.mark.parametrize("P,flux,prec", [(1, "lf", "fp64"), (3, "roe", "fp64"), (2, "roe", "fp64"),
+                                         (2, "lf", "fp32"), (3, "roe", "bf16")])
+def test_noalloc_two_steps_on_one_context(P, flux, prec):
+    print(f"case: step_noalloc two steps P{P} {flux} {prec}", flush=True)
     res = DeviceDoubles(2)
-    kw = dict(degree=P, flux=flux, dt_policy="max", cfl=0.3 if P < 3 else 0.15)
+    kw = dict(degree=P, flux=flux, dt_policy="max", cfl=0.3 if P < 3 else 0.15, precision=prec)
     Ur, rr, Un, rn, _, _ = reference_and_noalloc(vortex_grid(20, seed=3), kw, IC_VORTEX, vortex_params(), 2, res)
     assert np.array_equal(Ur, Un)
     assert np.array_equal(rr, rn), (rr, rn)
