@@ -557,3 +557,32 @@ def test_compute_dt_leaves_time_and_step_count():
     assert n1 == n0 + 1 and abs(t1 - (t0 + dt)) <= 1e-15 * t1
     o.advance(5)
     assert abs(o.compute_dt() - dt) <= 1e-13 * dt
+
+
+def test_tiles_whose_halo_does_not_fit_the_staged_image():
+    """A caller order that keeps the mesh's row-major numbering: 256 consecutive elements of a
+    64x64 split-quad mesh span two rows, so a tile's halo (about 130 elements) exceeds the 96
+    slots of the staged LDS image.  FP64 runs the global-gather tiles, the FP32 precision form
+    the four-lanes-per-element kernel, both against the oracle; BF16, which needs the staged
+    tiles, is refused with an error instead of running anything else."""
+    print("case: halo beyond the staged image (identity order, 64x64)", flush=True)
+    g = vortex_grid(64, jitter=0.2, diagonal=1, seed=17)
+    order = np.arange(g.n_elements, dtype=np.int32)
+    o, s = make_pair(g, P=2, flux="lf", dt_policy="max", cfl=0.3, order=order)
+    U0 = vortex_ic(o)
+    s.set_state(U0)
+    s.advance(5)
+    o.advance(5)
+    err = rel_l2(s.get_state(), o.get_state(), o.mass())
+    print("fp64 relative L2", err, flush=True)
+    assert np.all(err < TOL), err
+    s.close()
+    f = Solver(g, order=order, degree=2, flux="lf", dt_policy="max", cfl=0.3, precision="fp32")
+    f.set_state(U0)
+    f.advance(5)
+    err = rel_l2(f.get_state(), o.get_state(), o.mass())
+    print("fp32 relative L2", err, flush=True)
+    assert np.all(err < MX_TOL), err
+    f.close()
+    with pytest.raises(Exception, match="halo does not fit"):
+        Solver(g, order=order, degree=2, flux="lf", dt_policy="max", cfl=0.3, precision="bf16")
