@@ -213,7 +213,8 @@ struct LimArgs {
 constexpr int staged_block(int P) { return P <= 2 ? 256 : 128; }
 constexpr int staged_stride(int P) { return P <= 2 ? 352 : 200; }
 // Precision form: 256-element staged tiles at every degree; at P3 its LDS image holds FP32
-// modes and FP64 cell means (k_stage_pz), at P1/P2 the FP64 image of the FP64 form.
+// modes (FP32 form) or BF16 deviations (BF16 form) and FP64 cell means (k_stage_pz; the bound
+// below is the FP32 form's, the larger), at P1/P2 the FP64 image of the FP64 form.
 constexpr int px_block(int P) { return 256; }
 constexpr int px_stride(int P) { return 352; }
 constexpr long long px_lds_bytes(int P) {

@@ -1384,9 +1384,9 @@ template <int FLUX, int CT>
 static void launch_stage_pz_f(const StageArgs& a, hipStream_t s) {
     static bool opted = false;  // above 64 KiB the kernel must opt in
     if (!opted) {
-        opted = hipFuncSetAttribute((const void*)k_stage_pz<3, FLUX, CT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Pz<3>::LDS) == hipSuccess;
+        opted = hipFuncSetAttribute((const void*)k_stage_pz<3, FLUX, CT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Pz<3>::lds(CT)) == hipSuccess;
     }
-    hipLaunchKernelGGL((k_stage_pz<3, FLUX, CT>), dim3(grid_blocks(a)), dim3(Pz<3>::B), Pz<3>::LDS, s, a);
+    hipLaunchKernelGGL((k_stage_pz<3, FLUX, CT>), dim3(grid_blocks(a)), dim3(Pz<3>::B), Pz<3>::lds(CT), s, a);
 }
 
 int launch_stage(int P, int flux, int prec, int block, const StageArgs& a, void* stream) {
