@@ -308,15 +308,16 @@ def main():
     ap.add_argument("--cpu-steps-1t", type=int, default=3, help="oracle steps of the one-thread CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip every extra leg")
-    ap.add_argument("--extra", default="p2_2M,batch,batch_fp32",
+    ap.add_argument("--extra", default="p2_2M,batch,batch_fp32,riemann,p3",
                     help="comma-separated extra legs: p2_2M, batch, batch_fp32, batch_bf16, batch_p3, riemann, p3 "
-                         "(riemann runs 2 x ~25k steps to t = 0.3 and p3 builds an 8.4M-triangle mesh)")
+                         "(p3 builds an 8.4M-triangle mesh)")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-MFMA leg")
     ap.add_argument("--form", choices=("fp64", "fp32", "bf16"), default="fp64",
                     help="form of the main timed leg (fp32 / bf16: the precision form with FP32 or BF16 "
                          "contractions; the comparison legs are then skipped)")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batched leg")
-    ap.add_argument("--riemann-t-end", type=float, default=0.3, help="end time of the riemann leg")
+    ap.add_argument("--riemann-t-end", type=float, default=0.01,
+                    help="end time of the riemann leg (0.3, the problem's own, takes about 2 x 70k steps)")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the stream-copy measurement")
     args = ap.parse_args()
     if args.gpus != 1:
