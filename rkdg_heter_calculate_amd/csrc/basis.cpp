@@ -157,16 +157,9 @@ void build_ref_basis(int P, RefBasis& B) {
         T.recon[0][k] = (a22 * A[k][0] - a12 * A[k][1]) / det;
         T.recon[1][k] = (a11 * A[k][1] - a12 * A[k][0]) / det;
     }
-    // constant-flux sums of the volume and edge rules (same operation order as a rolled
-    // accumulation over the points)
+    // the edge rule applied to a constant flux (same operation order as a rolled accumulation
+    // over the points)
     for (int i = 0; i < Np; ++i) {
-        double gx = 0.0, ge = 0.0;
-        for (int q = 0; q < nq; ++q) {
-            gx += T.wv[q] * T.Dxi[q][i];
-            ge += T.wv[q] * T.Deta[q][i];
-        }
-        T.Gxi[i] = gx;
-        T.Geta[i] = ge;
         for (int k = 0; k < 3; ++k) {
             double se = 0.0;
             for (int j = 0; j < ne; ++j) se += T.we[j] * T.Ve[k][j][i];

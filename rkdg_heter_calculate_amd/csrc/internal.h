@@ -63,11 +63,10 @@ struct DevTables {
     double minv[MAXNP];            // 1 / (quadrature mass per unit area)
     double Vmid[3][3];             // P1 modes at the edge midpoints (limiter)
     double recon[2][3];            // modes 1,2 from midpoint deviations (limiter)
-    // the volume and edge rules applied to a constant flux (the precision forms' mean-state
-    // part: in FP64 at P1/P2; Se in the edge corrections at P3): sum_q w_q d psi_i / d xi
-    // (eta), and sum_j we_j psi_i(edge k point j)
-    double Gxi[MAXNP];
-    double Geta[MAXNP];
+    // the edge rule applied to a constant flux, sum_j we_j psi_i(edge k point j): the BF16
+    // precision forms' edge corrections (for a constant flux the volume rule's
+    // A (F.xi sum_q w_q d psi_i/d xi + F.eta sum_q w_q d psi_i/d eta) equals
+    // sum_k l_k/2 F.n_k Se[k][i], so the precision forms leave both out)
     double Se[3][MAXNP];
 };
 
